@@ -1,0 +1,66 @@
+"""ctypes binding of libarpeggio_hip.so (the C ABI declared in include/arpeggio_hip.h).
+
+The library is built in-tree by arpeggio_amd.build. There is no fallback: if the library or a GPU is missing, the
+entry points raise."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libarpeggio_hip.so')
+
+_P = ctypes.c_void_p
+_i32, _u32, _u64 = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64
+
+# name -> argtypes; every entry point returns int (0 = success)
+SIGNATURES = {
+    'arp_philox_kat': [_P, _P, _i32, _P],
+    'arp_anneal_arpeggio': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _i32, _P, _P, _P,
+                            _P],
+    'arp_anneal_chord': [_P, _P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _i32, _P, _P, _P,
+                         _P],
+}
+
+_lib = None
+
+
+class ArpeggioError(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Loads the library (once). Raises ArpeggioError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ArpeggioError('%s is not built: run arpeggio_amd.build.build() (hipcc, gfx950)' % p)
+    L = ctypes.CDLL(p)
+    L.arp_last_error.restype = ctypes.c_char_p
+    L.arp_last_error.argtypes = []
+    L.arp_abi_version.restype = ctypes.c_int
+    for name, args in SIGNATURES.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().arp_last_error().decode(errors='replace')
+        if rc == -1 or rc == -3:
+            raise ValueError('%s: %s' % (what, msg))
+        raise ArpeggioError('%s failed (%d): %s' % (what, rc, msg))
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)

@@ -1,0 +1,104 @@
+// C ABI entry points of libarpeggio_hip.so: argument checks, launch-shape choice, error reporting.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <string>
+
+#include "../../include/arpeggio_hip.h"
+#include "common.h"
+
+namespace arp {
+hipError_t launch_arpeggio_fx(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R, int D,
+                              int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode,
+                              int8_t* out_s, long long* out_de, long long* out_st, int waves, hipStream_t st);
+hipError_t launch_chord_fx(const int16_t* J, const int32_t* h, const int32_t* h0, const int32_t* Tq,
+                           const int32_t* Cq, int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1,
+                           int inst0, int mode, int32_t* out_e, int8_t* out_s, unsigned long long* stats, int waves,
+                           size_t max_lds, hipStream_t st);
+
+__global__ void philox_kat_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t c0 = in[6 * t], c1 = in[6 * t + 1], c2 = in[6 * t + 2], c3 = in[6 * t + 3];
+  philox10(c0, c1, c2, c3, in[6 * t + 4], in[6 * t + 5]);
+  out[4 * t] = c0;
+  out[4 * t + 1] = c1;
+  out[4 * t + 2] = c2;
+  out[4 * t + 3] = c3;
+}
+}  // namespace arp
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+static int ok() {
+  g_err.clear();
+  return ARP_OK;
+}
+
+static int hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) return fail(ARP_E_HIP, "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+  return ok();
+}
+
+static size_t device_max_lds() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) return 65536;
+  return (size_t)v;
+}
+
+extern "C" const char* arp_last_error(void) { return g_err.c_str(); }
+
+extern "C" int arp_abi_version(void) { return 1; }
+
+extern "C" int arp_philox_kat(const uint32_t* in, uint32_t* out, int n, void* stream) {
+  if (n < 0) return fail(ARP_E_ARG, "philox_kat: n = %d < 0", n);
+  if (n == 0) return ok();
+  if (!in || !out) return fail(ARP_E_ARG, "philox_kat: null pointer");
+  hipLaunchKernelGGL(arp::philox_kat_kernel, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, in, out, n);
+  return hip_check(hipGetLastError(), "philox_kat launch");
+}
+
+extern "C" int arp_anneal_arpeggio(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R,
+                                   int D, int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode,
+                                   int8_t* out_s, int64_t* out_de, int64_t* out_st, void* stream) {
+  if (P != 2 && P != 4 && P != 6) return fail(ARP_E_ARG, "arpeggio: P = %d, expected 2, 4 or 6", P);
+  if (D < 1 || D > 128) return fail(ARP_E_RANGE, "arpeggio: D = %d outside 1..128", D);
+  if (L < 1) return fail(ARP_E_ARG, "arpeggio: L = %d, need at least one pass", L);
+  if (R < 1) return fail(ARP_E_ARG, "arpeggio: R = %d < 1", R);
+  if (B < 0) return fail(ARP_E_ARG, "arpeggio: B = %d < 0", B);
+  if (start != 0 && start != 1) return fail(ARP_E_ARG, "arpeggio: start = %d, expected 0 (rand) or 1 (bal)", start);
+  if (mode != 0 && mode != 1) return fail(ARP_E_ARG, "arpeggio: mode = %d, expected 0 or 1", mode);
+  if (B == 0) return ok();
+  if (!Gt || !h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio: null pointer");
+  const int waves = R < 4 ? R : 4;
+  return hip_check(arp::launch_arpeggio_fx(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, start, mode, out_s,
+                                           (long long*)out_de, (long long*)out_st, waves, (hipStream_t)stream),
+                   "arpeggio launch");
+}
+
+extern "C" int arp_anneal_chord(const int16_t* J, const int32_t* h, const int32_t* h0, const int32_t* Tq,
+                                const int32_t* Cq, int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1,
+                                int inst0, int mode, int32_t* out_e, int8_t* out_s, uint64_t* stats, void* stream) {
+  if (N < 32 || N > 256 || N % 32) return fail(ARP_E_RANGE, "chord: N = %d, expected a multiple of 32 up to 256", N);
+  if (P < 1 || N % P) return fail(ARP_E_ARG, "chord: P = %d does not divide N = %d", P, N);
+  if (R < 1) return fail(ARP_E_ARG, "chord: R = %d < 1", R);
+  if (S < 0 || B < 0) return fail(ARP_E_ARG, "chord: S = %d, B = %d", S, B);
+  if (mode != 0 && mode != 1) return fail(ARP_E_ARG, "chord: mode = %d, expected 0 or 1", mode);
+  if (B == 0) return ok();
+  if (!J || !h || !h0 || (S > 0 && (!Tq || !Cq)) || !out_e || !out_s) return fail(ARP_E_ARG, "chord: null pointer");
+  const int waves = R < 4 ? R : 4;
+  return hip_check(arp::launch_chord_fx(J, h, h0, Tq, Cq, q, S, P, R, N, B, k0, k1, inst0, mode, out_e, out_s,
+                                        (unsigned long long*)stats, waves, device_max_lds(), (hipStream_t)stream),
+                   "chord launch");
+}

@@ -1,0 +1,66 @@
+/*
+ * arpeggio_hip — C ABI of the MI355X (gfx950) kernels for the delta-encoded Ising annealing detector.
+ *
+ * Every entry point takes device pointers (memory the caller allocated, e.g. PyTorch-ROCm tensors), plain sizes and
+ * a HIP stream (hipStream_t passed as void*; NULL = the default stream). Launches are asynchronous on that stream;
+ * nothing is allocated or freed here. Each returns 0 on success or a negative ARP_E* code, with a message available
+ * from arp_last_error() (per calling thread). Calls are thread-compatible: distinct streams may be driven from
+ * distinct threads.
+ *
+ * Integer ranges (the reference kernels share them): fields are int32, so for the Arpeggio kernel
+ * max_i(|h_i| + 2 P sum_j |G_ij|) + 2 max|G_jj| + |q| + 2 max T must stay below 2^31, and 0 <= T < 2^29.
+ * The Python layer (arpeggio_amd) checks these before a launch.
+ */
+#ifndef ARPEGGIO_HIP_H
+#define ARPEGGIO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARP_OK 0
+#define ARP_E_ARG (-1)    /* invalid argument (sizes, pointers, unsupported P / D / N) */
+#define ARP_E_HIP (-2)    /* a HIP runtime call failed */
+#define ARP_E_RANGE (-3)  /* a size beyond what the kernel supports */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *arp_last_error(void);
+
+/* ABI version of this library (incremented when a signature changes). */
+int arp_abi_version(void);
+
+/* Philox4x32-10 known-answer kernel: in (n, 6) uint32 rows (counter[4], key[2]) -> out (n, 4).
+ * Replaces the reference's device KAT `philox_kat` (arpeggio/mimo/chord_kernel.cu:26-32), driven by
+ * Kernel.philox_kat (arpeggio/mimo/chordfx.py:180-187). */
+int arp_philox_kat(const uint32_t *in, uint32_t *out, int n, void *stream);
+
+/* Fixed-point Arpeggio first-accept annealer (mode 0 'arpeggio'; mode 1 'par', the engine's parallel pass).
+ * Replaces `arpeggio_fx` (arpeggio/sim/arpeggio_fx.cu:35-37, same argument order and meaning; launched by
+ * arpeggio/sim/arpeggiofx.py:23-44) and the 'par' mode of arpeggio/sim/local_code_a59/arpeggio_ref59.py:41-85.
+ *   Gt    (B, D, D) int32, transposed Gram: Gt[b][j][i] = G_ij     h (B, D) int32      T (L) int32, one per pass
+ *   q     pinning, Q units    P copies in {2, 4, 6}    R anneals per frame    1 <= D <= 128
+ *   inst  (B) uint32 Philox instance words    k0, k1 = low, high 32 bits of the 64-bit key
+ *   start 0 'rand', 1 'bal'
+ *   out_s (B, R, P*D) int8 best spins, copy-major (spin p*D + j)   out_de (B, R) int64 best energy change (2 E0, Q)
+ *   out_st (B, R, 2) int64 (flips, tests) */
+int arp_anneal_arpeggio(const int32_t *Gt, const int32_t *h, const int32_t *T, int L, int q, int P, int R, int D,
+                        int B, const uint32_t *inst, uint32_t k0, uint32_t k1, int start, int mode, int8_t *out_s,
+                        int64_t *out_de, int64_t *out_st, void *stream);
+
+/* Fixed-point Chord annealer. Replaces `chord_m{1,2,3,4,6,8}` (arpeggio/mimo/chord_kernel.cu:177-184, same argument
+ * order and meaning; launched by Kernel.chord, arpeggio/mimo/chordfx.py:189-208), bit-exact to chordfx.chord_fx.
+ *   J (B, N, N) int16 symmetric   h, h0 (B, N) int32   Tq, Cq (S) int32   q pinning   P phases (copies), N % P == 0
+ *   R anneals   N <= 256, N % 32 == 0   k0, k1 seed words   inst0: frame b uses Philox instance inst0 + b
+ *   mode 0: out_e (B, R) int32, out_s (B, R, N) int8;  mode 1: out_e (B,), out_s (B, N) (lowest energy, lowest r)
+ *   stats: NULL or 4 uint64 accumulators (flips, energy evaluations, Philox blocks, steps x anneals) */
+int arp_anneal_chord(const int16_t *J, const int32_t *h, const int32_t *h0, const int32_t *Tq, const int32_t *Cq,
+                     int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1, int inst0, int mode,
+                     int32_t *out_e, int8_t *out_s, uint64_t *stats, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARPEGGIO_HIP_H */

@@ -1,0 +1,40 @@
+"""The C-ABI library loads and exports every entry point include/arpeggio_hip.h declares (no GPU work)."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def declared():
+    txt = open(os.path.join(ROOT, 'include', 'arpeggio_hip.h')).read()
+    return sorted(set(re.findall(r'^\s*(?:const\s+)?\w+\s*\*?\s*(arp_\w+)\s*\(', txt, re.M)))
+
+
+def test_header_symbols_exported():
+    from arpeggio_amd import _lib
+    names = declared()
+    assert len(names) >= 5
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    from arpeggio_amd import _lib
+    names = set(declared()) - {'arp_last_error', 'arp_abi_version'}
+    assert names <= set(_lib.SIGNATURES), names - set(_lib.SIGNATURES)
+    L = _lib.load()
+    assert L.arp_abi_version() >= 1
+    assert L.arp_last_error() == b''
+
+
+def test_argument_errors_without_gpu():
+    """Invalid sizes are refused before any HIP call, with a message."""
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    rc = L.arp_anneal_arpeggio(None, None, None, 8, 0, 3, 2, 16, 1, None, 0, 0, 0, 0, None, None, None, None)
+    assert rc == -1 and b'P = 3' in L.arp_last_error()
+    rc = L.arp_anneal_chord(None, None, None, None, None, 0, 8, 2, 2, 48, 1, 0, 0, 0, 0, None, None, None, None)
+    assert rc == -3
+    assert L.arp_anneal_arpeggio(None, None, None, 8, 0, 2, 2, 16, 0, None, 0, 0, 0, 0, None, None, None, None) == 0
