@@ -12,7 +12,7 @@
 // de, best, flips and tests are wave-uniform and live in scalar registers; spins are one bit per copy per lane.
 // Each (lane, copy) Philox block gives the words of 4 consecutive sweeps, so a block is computed every 4th sweep of
 // a copy instead of once per test.
-#include "common.h"
+#include "kernels.h"
 
 namespace arp {
 

@@ -11,7 +11,7 @@
 // ballot), so the copy test, the snapshot commit (masked ballot + find-first-set per committed flip), the energy and
 // the best state are scalar work; fields u are per lane. A committed flip of spin j reads row j of J (= column j)
 // once, N consecutive int16 words, and updates every lane's fields.
-#include "common.h"
+#include "kernels.h"
 
 namespace arp {
 

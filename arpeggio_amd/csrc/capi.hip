@@ -5,16 +5,9 @@
 #include <string>
 
 #include "../../include/arpeggio_hip.h"
-#include "common.h"
+#include "kernels.h"
 
 namespace arp {
-hipError_t launch_arpeggio_fx(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R, int D,
-                              int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode,
-                              int8_t* out_s, long long* out_de, long long* out_st, int waves, hipStream_t st);
-hipError_t launch_chord_fx(const int16_t* J, const int32_t* h, const int32_t* h0, const int32_t* Tq,
-                           const int32_t* Cq, int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1,
-                           int inst0, int mode, int32_t* out_e, int8_t* out_s, unsigned long long* stats, int waves,
-                           size_t max_lds, hipStream_t st);
 
 __global__ void philox_kat_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,4 +94,36 @@ extern "C" int arp_anneal_chord(const int16_t* J, const int32_t* h, const int32_
   return hip_check(arp::launch_chord_fx(J, h, h0, Tq, Cq, q, S, P, R, N, B, k0, k1, inst0, mode, out_e, out_s,
                                         (unsigned long long*)stats, waves, device_max_lds(), (hipStream_t)stream),
                    "chord launch");
+}
+
+extern "C" int arp_front_end(const float* H, const float* y, const float* N0, float N0_scalar, int B, int Mr, int D,
+                             int M, int K, float Q, int8_t* xg, int32_t* Gi, int32_t* hi, float* scale, float* Gf,
+                             float* hf, int16_t* J, int32_t* hN, float* z, float* mu, void* stream) {
+  if (D < 1 || D > 128) return fail(ARP_E_RANGE, "front_end: D = %d outside 1..128", D);
+  if (Mr < 1 || Mr > 128) return fail(ARP_E_RANGE, "front_end: Mr = %d outside 1..128", Mr);
+  if (M != 4 && M != 16) return fail(ARP_E_ARG, "front_end: M = %d, expected 4 or 16", M);
+  if (K < 1 || K > 3) return fail(ARP_E_ARG, "front_end: K = %d outside 1..3", K);
+  if (J && 2 * K * D > 256) return fail(ARP_E_RANGE, "front_end: expanded N = %d > 256", 2 * K * D);
+  if (B < 0) return fail(ARP_E_ARG, "front_end: B = %d < 0", B);
+  if (B == 0) return ok();
+  if (!H || !y) return fail(ARP_E_ARG, "front_end: null input");
+  arp::FrontOut o{xg, Gi, hi, scale, Gf, hf, J, hN, z, mu};
+  return hip_check(arp::launch_front_end(H, y, N0, N0_scalar, B, Mr, D, M, K, Q, o, (hipStream_t)stream),
+                   "front_end launch");
+}
+
+extern "C" int arp_decode_select(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R,
+                                 int P, int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat,
+                                 int32_t* biterr, float* dist, void* stream) {
+  if (D < 1 || D > 128 || Mr < 1 || Mr > 256) return fail(ARP_E_RANGE, "decode: D = %d, Mr = %d", D, Mr);
+  if (R < 0 || R > 256) return fail(ARP_E_RANGE, "decode: R = %d outside 0..256", R);
+  if (M != 4 && M != 16) return fail(ARP_E_ARG, "decode: M = %d", M);
+  if (delta && (P < 1 || P > 6)) return fail(ARP_E_ARG, "decode: P = %d", P);
+  if (B < 0) return fail(ARP_E_ARG, "decode: B = %d < 0", B);
+  if (B == 0) return ok();
+  if (!H || !y || !xg || (R > 0 && !spins)) return fail(ARP_E_ARG, "decode: null input");
+  if (biterr && !ltrue) return fail(ARP_E_ARG, "decode: biterr needs ltrue");
+  return hip_check(arp::launch_decode_select(H, y, xg, spins, B, R, P, D, Mr, M, delta, ltrue, lhat, biterr, dist,
+                                             (hipStream_t)stream),
+                   "decode launch");
 }

@@ -1,0 +1,155 @@
+"""The detector's batched kernel path on MI355X: front end -> anneal -> decode/select, all HIP kernels.
+
+  front_end(H, y, N0, M, K)            LMMSE hard decision + dDI Ising build + fixed point (arp_front_end)
+  decode_select(H, y, xg, spins, ...)  candidates -> lowest ||y - Hx||^2 -> bits (arp_decode_select)
+  DetectPipeline                       preallocated buffers; run() launches the three kernels back to back on one
+                                       stream with no host synchronisation (capturable in a HIP graph)
+  detect(H, y, sigma2, policy, copies, cycles, seed)
+                                       the one-call detector API (sigma2 = N0/2 per real dimension)
+
+Reference call stack this replaces: arpeggio/mimo/fxdetect.py:24-43 (MMSE -> mapping -> to_fixed -> kernel ->
+levels -> select) with the fixed-point Arpeggio solver of arpeggio/sim/local_code_a513/run_a513.py:107-128."""
+import torch
+
+from . import _lib
+from .chordfx import QSCALE, geom, to_fixed
+
+MASK = 0xFFFFFFFF
+
+
+def _f32(x):
+    return x.to(device='cuda', dtype=torch.float32).contiguous()
+
+
+def _n0_arg(N0, B):
+    if torch.is_tensor(N0) and N0.numel() > 1:
+        t = N0.to(device='cuda', dtype=torch.float32).contiguous()
+        assert t.numel() == B
+        return t, 0.0
+    return None, float(N0)
+
+
+def front_end(H, y, N0, M, K=1, Q=QSCALE, want=('xg', 'Gi', 'hi', 'scale')):
+    """H (B, Mr, D), y (B, Mr) device float tensors; N0 scalar or (B,). Returns a dict with the requested outputs:
+    xg (B, D) int8, Gi (B, D, D) int32, hi (B, D) int32, scale (B,), Gf, hf (float G/scale, h/scale), J (B, N, N)
+    int16 and hN (B, N) int32 (N = 2K D), z, mu (B, D)."""
+    H, y = _f32(H), _f32(y)
+    B, Mr, D = H.shape
+    N = 2 * K * D
+    dev = H.device
+    shapes = dict(xg=((B, D), torch.int8), Gi=((B, D, D), torch.int32), hi=((B, D), torch.int32),
+                  scale=((B,), torch.float32), Gf=((B, D, D), torch.float32), hf=((B, D), torch.float32),
+                  J=((B, N, N), torch.int16), hN=((B, N), torch.int32), z=((B, D), torch.float32),
+                  mu=((B, D), torch.float32))
+    out = {k: torch.empty(*shapes[k][0], dtype=shapes[k][1], device=dev) for k in want}
+    n0t, n0s = _n0_arg(N0, B)
+    order = ('xg', 'Gi', 'hi', 'scale', 'Gf', 'hf', 'J', 'hN', 'z', 'mu')
+    L = _lib.load()
+    _lib.check(L.arp_front_end(_lib.ptr(H), _lib.ptr(y), _lib.ptr(n0t), n0s, B, Mr, D, int(M), int(K), float(Q),
+                               *[_lib.ptr(out.get(k)) for k in order], _lib.stream_ptr()), 'front_end')
+    return out
+
+
+def decode_select(H, y, xg, spins, M, P=2, delta=True, ltrue=None, want_dist=False):
+    """Returns (lhat (B, D) int8, biterr (B,) int32 or None, dist (B, R+1) or None)."""
+    H, y = _f32(H), _f32(y)
+    B, Mr, D = H.shape
+    R = spins.shape[1]
+    lhat = torch.empty(B, D, dtype=torch.int8, device=H.device)
+    be = torch.empty(B, dtype=torch.int32, device=H.device) if ltrue is not None else None
+    dist = torch.empty(B, R + 1, dtype=torch.float32, device=H.device) if want_dist else None
+    lt = ltrue.to(device='cuda', dtype=torch.int8).contiguous() if ltrue is not None else None
+    L = _lib.load()
+    _lib.check(L.arp_decode_select(_lib.ptr(H), _lib.ptr(y), _lib.ptr(xg.contiguous()), _lib.ptr(spins.contiguous()),
+                                   B, R, P, D, Mr, int(M), 1 if delta else 0, _lib.ptr(lt), _lib.ptr(lhat),
+                                   _lib.ptr(be), _lib.ptr(dist), _lib.stream_ptr()), 'decode_select')
+    return lhat, be, dist
+
+
+def arpeggio_schedule(S, T0, T1, q, K=1, Q=QSCALE):
+    """One integer T per pass, S sweeps x 2K passes, and q in Q units (run_a513.py:83-87)."""
+    Tsw = [int(round(v * Q)) for v in geom(T0, T1, S).double().tolist()]
+    return [t for t in Tsw for _ in range(2 * K)], int(round(q * Q))
+
+
+class DetectPipeline:
+    """One-stage dDI (K = 1) detection of a fixed-shape batch with preallocated device buffers.
+
+    policy 'arpeggio': first-accept anneal, cycles = L passes (S = L / 2 sweeps), schedule (T0, T1, q, start).
+    policy 'chord': snapshot anneal on the expanded N = 2D problem, cycles = S steps, schedule (t0, t1, q, kappa)."""
+
+    def __init__(self, B, Mr, D, M, R, policy='arpeggio', cycles=64, T0=0.1, T1=0.1, q=0.0, kappa=0.0,
+                 start='rand', key=0, inst0=0, mode='arpeggio', Q=QSCALE):
+        self.B, self.Mr, self.D, self.M, self.R = B, Mr, D, M, R
+        self.policy, self.K, self.P = policy, 1, 2
+        self.key, self.inst0, self.start, self.mode = key, inst0, start, mode
+        dev = 'cuda'
+        self.lib = _lib.load()
+        self.xg = torch.empty(B, D, dtype=torch.int8, device=dev)
+        self.scale = torch.empty(B, dtype=torch.float32, device=dev)
+        self.lhat = torch.empty(B, D, dtype=torch.int8, device=dev)
+        self.biterr = torch.empty(B, dtype=torch.int32, device=dev)
+        N = self.P * D
+        if policy == 'arpeggio':
+            assert cycles % self.P == 0, 'arpeggio cycles = passes, a multiple of P'
+            T, qi = arpeggio_schedule(cycles // self.P, T0, T1, q, self.K, Q)
+            self.T = torch.tensor(T, dtype=torch.int32, device=dev)
+            self.q = qi
+            self.Gi = torch.empty(B, D, D, dtype=torch.int32, device=dev)
+            self.hi = torch.empty(B, D, dtype=torch.int32, device=dev)
+            self.inst = ((torch.arange(B, dtype=torch.int64) + inst0) & MASK).to(torch.int32).to(dev)
+            self.spins = torch.empty(B, R, N, dtype=torch.int8, device=dev)
+            self.de = torch.empty(B, R, dtype=torch.int64, device=dev)
+            self.st = torch.empty(B, R, 2, dtype=torch.int64, device=dev)
+        elif policy == 'chord':
+            assert N <= 256
+            dummy = dict(J=torch.zeros(1, 2, 2), h=torch.zeros(1, 2), h0=torch.zeros(1, 2))
+            F = to_fixed(dummy, cycles, T0, T1, q, kappa)
+            self.T, self.c, self.q, self.S = F['T'].to(dev), F['c'].to(dev), F['q'], cycles
+            self.J = torch.empty(B, N, N, dtype=torch.int16, device=dev)
+            self.hN = torch.empty(B, N, dtype=torch.int32, device=dev)
+            self.spins = torch.empty(B, R, N, dtype=torch.int8, device=dev)
+            self.e = torch.empty(B, R, dtype=torch.int32, device=dev)
+        else:
+            raise ValueError('policy must be arpeggio or chord')
+        self.Q = float(Q)
+
+    def run(self, H, y, N0, ltrue=None, stream=None):
+        """Launches front end, anneal and decode on one stream; returns (lhat, biterr or None) device tensors."""
+        L, s = self.lib, _lib.stream_ptr(stream)
+        B, Mr, D, M, R, P = self.B, self.Mr, self.D, self.M, self.R, self.P
+        n0t, n0s = _n0_arg(N0, B)
+        P_ = _lib.ptr
+        if self.policy == 'arpeggio':
+            _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
+                                       P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None, s),
+                       'front_end')
+            _lib.check(L.arp_anneal_arpeggio(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D, B,
+                                             P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
+                                             0 if self.start == 'rand' else 1, 0 if self.mode == 'arpeggio' else 1,
+                                             P_(self.spins), P_(self.de), P_(self.st), s), 'arpeggio')
+        else:
+            _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg), None,
+                                       None, P_(self.scale), None, None, P_(self.J), P_(self.hN), None, None, s),
+                       'front_end')
+            _lib.check(L.arp_anneal_chord(P_(self.J), P_(self.hN), P_(self.hN), P_(self.T), P_(self.c), self.q, self.S,
+                                          P, R, P * D, B, self.key & MASK, (self.key >> 32) & MASK, self.inst0, 0,
+                                          P_(self.e), P_(self.spins), None, s), 'chord')
+        _lib.check(L.arp_decode_select(P_(H), P_(y), P_(self.xg), P_(self.spins), B, R, P, D, Mr, M, 1, P_(ltrue),
+                                       P_(self.lhat), P_(self.biterr) if ltrue is not None else None, None, s),
+                   'decode_select')
+        return self.lhat, (self.biterr if ltrue is not None else None)
+
+
+def detect(H, y, sigma2, policy='arpeggio', copies=16, cycles=64, seed=0, M=16, T0=0.1, T1=0.1, q=0.0, kappa=0.0,
+           start='rand', inst0=0, ltrue=None):
+    """One-call detector. H (B, 2Nr, 2Nt) real lift, y (B, 2Nr), sigma2 = N0/2 per real dimension (scalar or (B,)).
+    Returns levels (B, 2Nt) int8 (and per-frame bit errors if ltrue is given)."""
+    H, y = _f32(H), _f32(y)
+    B, Mr, D = H.shape
+    N0 = 2 * sigma2 if not torch.is_tensor(sigma2) else (2 * sigma2).to('cuda', torch.float32)
+    pipe = DetectPipeline(B, Mr, D, M, copies, policy=policy, cycles=cycles, T0=T0, T1=T1, q=q, kappa=kappa,
+                          start=start, key=seed, inst0=inst0)
+    lt = ltrue.to(device='cuda', dtype=torch.int8).contiguous() if ltrue is not None else None
+    lhat, be = pipe.run(H, y, N0, lt)
+    return (lhat, be) if ltrue is not None else lhat

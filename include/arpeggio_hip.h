@@ -59,6 +59,31 @@ int arp_anneal_chord(const int16_t *J, const int32_t *h, const int32_t *h0, cons
                      int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1, int inst0, int mode,
                      int32_t *out_e, int8_t *out_s, uint64_t *stats, void *stream);
 
+/* Fused front end, one workgroup per frame: LMMSE hard decision (Cholesky of H^T H + (N0/2)/E[l^2] I),
+ * delta-Ising build around it (dDI, 2K copies) on the matrix cores, and the fixed-point inputs of the annealers.
+ * Replaces, on the kernel path, mimo.mmse / mmse_hard / _mu / quantize_levels (arpeggio/mimo/mimo.py:86-109, 176-182),
+ * mimo.ising_delta_at (mimo.py:506-523), the J/h rounding of chordfx.to_fixed (arpeggio/mimo/chordfx.py:60-76) and
+ * chord_ref.compress (arpeggio/sim/x3model/chord_ref.py:47-59).
+ *   H (B, Mr, D) float32 real lift (Mr = 2 Nr, D = 2 Nt <= 128, Mr <= 128)    y (B, Mr) float32
+ *   N0: (B) float32 per-frame noise density, or NULL to use N0_scalar for every frame    M: 4 (QPSK) or 16 (16-QAM)
+ *   K: dDI radius (P = 2K copies)    Q: fixed-point scale (4096)
+ * Outputs (each may be NULL): xg (B, D) int8 levels; Gi (B, D, D) int32 = rne(G / scale Q) (symmetric, so also the
+ * transposed Gram the Arpeggio kernel takes); hi (B, D) int32; scale (B); Gf (B, D, D), hf (B, D) float32 G/scale,
+ * h/scale; J (B, N, N) int16 and hN (B, N) int32, the expanded Chord inputs (N = 2 K D <= 256); z, mu (B, D). */
+int arp_front_end(const float *H, const float *y, const float *N0, float N0_scalar, int B, int Mr, int D, int M, int K,
+                  float Q, int8_t *xg, int32_t *Gi, int32_t *hi, float *scale, float *Gf, float *hf, int16_t *J,
+                  int32_t *hN, float *z, float *mu, void *stream);
+
+/* Decode and select: candidate levels from anneal spins (delta = 1: clip(x_g + sum of the P copies); delta = 0: RI
+ * spins), lowest ||y - H x||^2 over the R candidates then x_g (first minimum), Gray bit errors against ltrue.
+ * Replaces mimo.spins_to_levels(_delta), mimo.dist, mimo.select, mimo.bit_errors (arpeggio/mimo/mimo.py:39-51,
+ * 185-190, 416-431) and arpx.biterr_per_use (arpeggio/sim/arpx.py:63-66).
+ *   spins (B, R, N) int8 (N = P D for delta = 1; D or 2 D for RI)    ltrue (B, D) int8 or NULL
+ *   lhat (B, D) int8, biterr (B) int32 (needs ltrue), dist (B, R + 1) float32; each may be NULL. */
+int arp_decode_select(const float *H, const float *y, const int8_t *xg, const int8_t *spins, int B, int R, int P, int D,
+                      int Mr, int M, int delta, const int8_t *ltrue, int8_t *lhat, int32_t *biterr, float *dist,
+                      void *stream);
+
 #ifdef __cplusplus
 }
 #endif

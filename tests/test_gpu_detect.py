@@ -1,0 +1,145 @@
+"""GPU parity of the front end (LMMSE + dDI Ising build + fixed point), decode/select, and the whole detection chain.
+
+Float stages are compared with the float64 oracle within float32 tolerances; the integer hand-off (G_int, h_int,
+x_g) must equal the oracle's rounding except where the oracle's value sits within 1e-3 of a rounding boundary; the
+anneal and decode stages are bit-exact on the integers the GPU front end produced."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print('[case]', *a, flush=True)
+
+
+def frames(nt, M, snr, B, seed=4242, bi=0, **kw):
+    from arpeggio_amd import frames as fr
+    return fr.draw(seed, snr, bi, B, nt, M, dev='cuda', **kw)
+
+
+def near_half(x, tol=1e-3):
+    return np.abs(np.abs(x - np.floor(x)) - 0.5) < tol
+
+
+@pytest.mark.parametrize('nt,M,snr,K', [(16, 16, 16.0, 1), (32, 16, 20.0, 1), (16, 4, 8.0, 1), (64, 16, 22.0, 1),
+                                        (16, 16, 14.0, 2), (8, 16, 12.0, 3)])
+def test_front_end_vs_oracle(orc, nt, M, snr, K):
+    from arpeggio_amd.detector import front_end
+    say('front_end', nt, M, snr, K)
+    b = frames(nt, M, snr, 64)
+    want = ('xg', 'Gi', 'hi', 'scale', 'Gf', 'hf', 'z', 'mu') + (('J', 'hN') if 2 * K * 2 * nt <= 256 else ())
+    o = front_end(b['H'], b['y'], b['N0'], M, K, want=want)
+    torch.cuda.synchronize()
+    H, y = b['H'].cpu().numpy(), b['y'].cpu().numpy()
+    z, Ainv, _ = orc.mmse(H, y, b['N0'], M)
+    assert np.allclose(o['z'].cpu().numpy(), z, rtol=1e-3, atol=1e-3)
+    Gm = np.swapaxes(H, 1, 2).astype(np.float64) @ H
+    mu = np.diagonal(Ainv @ Gm, axis1=1, axis2=2)
+    assert np.allclose(o['mu'].cpu().numpy(), mu, rtol=1e-4, atol=1e-5)
+    xg = orc.quantize_levels(z / mu, M)
+    v = z / mu
+    edge = np.abs(v - np.round(v)) < 1e-3       # z/mu on a decision boundary (an even integer)
+    got_xg = o['xg'].cpu().numpy()
+    assert ((got_xg == xg) | edge).all()
+    # Ising build on the GPU's own x_g (the integer hand-off point)
+    I = orc.ising_delta_at(H, y, got_xg.astype(np.float64), K)
+    assert np.allclose(o['scale'].cpu().numpy(), I['scale'], rtol=1e-5)
+    Gf, hf = o['Gf'].cpu().numpy(), o['hf'].cpu().numpy()
+    assert np.allclose(Gf, I['G'], atol=1e-5)
+    assert np.allclose(hf, I['hD'], atol=1e-5)
+    Gi, hi = o['Gi'].cpu().numpy(), o['hi'].cpu().numpy()
+    # the fixed point is the half-even rounding of the GPU's own scaled floats (x 2^12 is exact in float32)
+    assert (Gi == np.rint(Gf * np.float32(4096))).all() and (hi == np.rint(hf * np.float32(4096))).all()
+    assert (Gi == np.swapaxes(Gi, 1, 2)).all()
+    Gq, hq = I['G'] * 4096, I['hD'] * 4096
+    assert ((Gi == np.rint(Gq)) | near_half(Gq, 0.05)).all()
+    assert ((hi == np.rint(hq)) | near_half(hq, 0.05)).all()
+    if 'J' in o:
+        J, hN = orc.expand(Gi.astype(np.int64), hi, K)
+        assert (o['J'].cpu().numpy() == J).all() and (o['hN'].cpu().numpy() == hN).all()
+
+
+def test_front_end_golden(golden):
+    """The reference's own float32 front end (mimo.mmse / mmse_hard / ising_delta_at / to_fixed + compress) on its
+    seeded frames: the GPU's x_g and integer Gram agree except at rounding boundaries."""
+    from arpeggio_amd.detector import front_end
+    z, idx = golden
+    for tag, f in idx['front'].items():
+        if tag == 'tofixed':
+            continue
+        say('front golden', tag)
+        p = 'front/%s/' % tag
+        H, y = torch.as_tensor(z[p + 'H']).cuda(), torch.as_tensor(z[p + 'y']).cuda()
+        o = front_end(H, y, f['N0'], f['M'], f['K'], want=('xg', 'Gi', 'hi'))
+        torch.cuda.synchronize()
+        xg = o['xg'].cpu().numpy()
+        assert (xg == z[p + 'lm_hard']).mean() > 0.99
+        same = (xg == z[p + 'lm_hard']).all(1)
+        Gi, Gr = o['Gi'].cpu().numpy()[same], z[p + 'G'][same]
+        assert np.abs(Gi - Gr).max() <= 1 and (Gi == Gr).mean() > 0.999
+
+
+@pytest.mark.parametrize('R,delta', [(16, True), (1, True), (33, True)])
+def test_decode_select_vs_oracle(orc, R, delta):
+    from arpeggio_amd.detector import decode_select, front_end
+    say('decode', R, delta)
+    M, nt, B = 16, 32, 48
+    b = frames(nt, M, 18.0, B, seed=77)
+    o = front_end(b['H'], b['y'], b['N0'], M, 1, want=('xg',))
+    g = torch.Generator(device='cuda')
+    g.manual_seed(3)
+    spins = torch.where(torch.rand(B, R, 4 * nt, device='cuda', generator=g) < 0.5, -1, 1).to(torch.int8)
+    lt = b['l'].to(torch.int8)
+    lhat, be, dist = decode_select(b['H'], b['y'], o['xg'], spins, M, 2, delta, ltrue=lt, want_dist=True)
+    torch.cuda.synchronize()
+    H, y = b['H'].cpu().numpy(), b['y'].cpu().numpy()
+    xg = o['xg'].cpu().numpy()
+    Lc = orc.spins_to_levels_delta(spins.cpu().numpy(), xg, 1, M)
+    lh, allc, d = orc.select(H, y, Lc, xg)
+    dg = dist.cpu().numpy()
+    assert np.allclose(dg, d, rtol=1e-4, atol=1e-3)
+    pick = dg.argmin(1)                                   # numpy argmin: the first minimum, as the kernel's scan
+    lg = lhat.cpu().numpy()
+    assert (lg == allc[np.arange(B), pick]).all()
+    # the GPU's choice is the oracle's, or a near-tie of distinct candidates within float32 rounding
+    for i in range(B):
+        if not (lg[i] == lh[i]).all():
+            assert d[i, pick[i]] <= d[i].min() * (1 + 1e-5) + 1e-6
+    be_o = orc.biterr_per_use(b['l'].cpu().numpy(), lhat.cpu().numpy(), M)
+    assert (be.cpu().numpy() == be_o).all()
+
+
+@pytest.mark.parametrize('policy', ['arpeggio', 'chord'])
+def test_pipeline_chain_bit_exact(orc, policy):
+    """The whole chain (front end -> anneal -> decode) equals the oracle chain run on the GPU front end's integers."""
+    from arpeggio_amd.detector import DetectPipeline
+    M, nt, B, R = 16, 32, 40, 16
+    say('pipeline', policy)
+    b = frames(nt, M, 20.0, B, seed=11)
+    cyc = 64 if policy == 'arpeggio' else 128
+    pipe = DetectPipeline(B, 2 * nt, 2 * nt, M, R, policy=policy, cycles=cyc, T0=0.1 if policy == 'arpeggio' else 0.3,
+                          T1=0.1 if policy == 'arpeggio' else 0.2, q=0.0 if policy == 'arpeggio' else 0.4,
+                          kappa=0.0 if policy == 'arpeggio' else 0.3, key=0xDEADBEEF12345, inst0=5)
+    lt = b['l'].to(torch.int8)
+    lhat, be = pipe.run(b['H'], b['y'], b['N0'], lt)
+    torch.cuda.synchronize()
+    xg = pipe.xg.cpu().numpy()
+    if policy == 'arpeggio':
+        G, h = pipe.Gi.cpu().numpy(), pipe.hi.cpu().numpy()
+        s, de, st = orc.arpeggio_fx(G, h, 2, R, pipe.T.cpu().numpy(), pipe.q, np.arange(B) + 5, 0xDEADBEEF12345, 'rand')
+        assert (pipe.spins.cpu().numpy() == s).all() and (pipe.de.cpu().numpy() == de).all()
+        assert (pipe.st.cpu().numpy() == st).all()
+    else:
+        F = dict(J=pipe.J.cpu().numpy(), h=pipe.hN.cpu().numpy(), h0=pipe.hN.cpu().numpy(), T=pipe.T.cpu().numpy(),
+                 c=pipe.c.cpu().numpy(), q=pipe.q, S=pipe.S)
+        s, e = orc.chord_fx(F, R, 0xDEADBEEF12345, P=2, inst0=5)
+        assert (pipe.spins.cpu().numpy() == s).all() and (pipe.e.cpu().numpy() == e).all()
+    H, y = b['H'].cpu().numpy(), b['y'].cpu().numpy()
+    Lc = orc.spins_to_levels_delta(s, xg, 1, M)
+    lh, _, d = orc.select(H, y, Lc, xg)
+    agree = (lhat.cpu().numpy() == lh).all(1)
+    assert agree.mean() >= 0.97
+    be_o = orc.biterr_per_use(b['l'].cpu().numpy(), lhat.cpu().numpy(), M)
+    assert (be.cpu().numpy() == be_o).all()
