@@ -19,7 +19,11 @@ SIGNATURES = {
     'arp_anneal_chord': [_P, _P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _i32, _P, _P, _P,
                          _P],
     'arp_front_end': [_P, _P, _P, ctypes.c_float, _i32, _i32, _i32, _i32, _i32, ctypes.c_float, _P, _P, _P, _P, _P, _P,
-                      _P, _P, _P, _P, _P],
+                      _P, _P, _P, _P, _P, _P, _P],
+    'arp_anneal_arpeggio_f32': [_P, _P, _P, _i32, ctypes.c_float, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _P, _P,
+                                _P, _P, _P],
+    'arp_anneal_chord_f32': [_P, _P, _P, _P, _P, ctypes.c_float, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _P, _P,
+                             _P, _P, _P],
     'arp_decode_select': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _P, _P, _P, _P, _P],
 }
 

@@ -66,3 +66,26 @@ def solve(G, h, P, R, T, q, inst, key, start, mode='arpeggio'):
     s, de, st = solve_device(G, h, P, R, T, q, inst, key, start, mode=mode)
     torch.cuda.synchronize()
     return s.cpu().numpy(), de.cpu().numpy(), st.cpu().numpy()
+
+
+def solve_float_device(G, h, P, R, T, q, inst, key, start, want_fields=False, transposed=False):
+    """FP32 twin of solve_device: G (B, D, D), h (B, D) float scaled couplings, T (L,) float temperatures per pass,
+    q float. Returns device tensors (spins int8, best dE float32 (E0 units), st int64 (B, R, 2), fields or None)."""
+    G = torch.as_tensor(G)
+    h = torch.as_tensor(h)
+    B, D, _ = G.shape
+    assert D <= 128 and P in (2, 4, 6) and len(T) >= 1 and start in ('rand', 'bal')
+    Gt = (G if transposed else G.transpose(1, 2)).to(device='cuda', dtype=torch.float32).contiguous()
+    hh = h.to(device='cuda', dtype=torch.float32).contiguous()
+    Tt = torch.as_tensor(T, dtype=torch.float32).to('cuda').contiguous()
+    it = torch.as_tensor(np.asarray(torch.as_tensor(inst).cpu().numpy(), dtype=np.int64) & M32).to(torch.int32).cuda()
+    out_s = torch.empty(B, R, P * D, dtype=torch.int8, device='cuda')
+    out_de = torch.empty(B, R, dtype=torch.float32, device='cuda')
+    out_st = torch.empty(B, R, 2, dtype=torch.int64, device='cuda')
+    out_w = torch.empty(B, R, D, dtype=torch.float32, device='cuda') if want_fields else None
+    L = _lib.load()
+    _lib.check(L.arp_anneal_arpeggio_f32(_lib.ptr(Gt), _lib.ptr(hh), _lib.ptr(Tt), Tt.numel(), float(q), int(P),
+                                         int(R), int(D), int(B), _lib.ptr(it), key & M32, (key >> 32) & M32,
+                                         0 if start == 'rand' else 1, _lib.ptr(out_s), _lib.ptr(out_de),
+                                         _lib.ptr(out_st), _lib.ptr(out_w), _lib.stream_ptr()), 'arpeggio_f32')
+    return out_s, out_de, out_st, out_w

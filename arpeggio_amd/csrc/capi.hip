@@ -98,16 +98,17 @@ extern "C" int arp_anneal_chord(const int16_t* J, const int32_t* h, const int32_
 
 extern "C" int arp_front_end(const float* H, const float* y, const float* N0, float N0_scalar, int B, int Mr, int D,
                              int M, int K, float Q, int8_t* xg, int32_t* Gi, int32_t* hi, float* scale, float* Gf,
-                             float* hf, int16_t* J, int32_t* hN, float* z, float* mu, void* stream) {
+                             float* hf, int16_t* J, int32_t* hN, float* z, float* mu, float* Jf, float* hNf,
+                             void* stream) {
   if (D < 1 || D > 128) return fail(ARP_E_RANGE, "front_end: D = %d outside 1..128", D);
   if (Mr < 1 || Mr > 128) return fail(ARP_E_RANGE, "front_end: Mr = %d outside 1..128", Mr);
   if (M != 4 && M != 16) return fail(ARP_E_ARG, "front_end: M = %d, expected 4 or 16", M);
   if (K < 1 || K > 3) return fail(ARP_E_ARG, "front_end: K = %d outside 1..3", K);
-  if (J && 2 * K * D > 256) return fail(ARP_E_RANGE, "front_end: expanded N = %d > 256", 2 * K * D);
+  if ((J || Jf) && 2 * K * D > 256) return fail(ARP_E_RANGE, "front_end: expanded N = %d > 256", 2 * K * D);
   if (B < 0) return fail(ARP_E_ARG, "front_end: B = %d < 0", B);
   if (B == 0) return ok();
   if (!H || !y) return fail(ARP_E_ARG, "front_end: null input");
-  arp::FrontOut o{xg, Gi, hi, scale, Gf, hf, J, hN, z, mu};
+  arp::FrontOut o{xg, Gi, hi, scale, Gf, hf, J, hN, z, mu, Jf, hNf};
   return hip_check(arp::launch_front_end(H, y, N0, N0_scalar, B, Mr, D, M, K, Q, o, (hipStream_t)stream),
                    "front_end launch");
 }
@@ -126,4 +127,33 @@ extern "C" int arp_decode_select(const float* H, const float* y, const int8_t* x
   return hip_check(arp::launch_decode_select(H, y, xg, spins, B, R, P, D, Mr, M, delta, ltrue, lhat, biterr, dist,
                                              (hipStream_t)stream),
                    "decode launch");
+}
+
+extern "C" int arp_anneal_arpeggio_f32(const float* Gt, const float* h, const float* T, int L, float q, int P, int R,
+                                       int D, int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start,
+                                       int8_t* out_s, float* out_de, int64_t* out_st, float* out_w, void* stream) {
+  if (P != 2 && P != 4 && P != 6) return fail(ARP_E_ARG, "arpeggio_f32: P = %d, expected 2, 4 or 6", P);
+  if (D < 1 || D > 128) return fail(ARP_E_RANGE, "arpeggio_f32: D = %d outside 1..128", D);
+  if (L < 1 || R < 1 || B < 0) return fail(ARP_E_ARG, "arpeggio_f32: L = %d, R = %d, B = %d", L, R, B);
+  if (start != 0 && start != 1) return fail(ARP_E_ARG, "arpeggio_f32: start = %d", start);
+  if (B == 0) return ok();
+  if (!Gt || !h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio_f32: null pointer");
+  const int waves = R < 4 ? R : 4;
+  return hip_check(arp::launch_arpeggio_f32(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, start, out_s, out_de,
+                                            (long long*)out_st, out_w, waves, (hipStream_t)stream),
+                   "arpeggio_f32 launch");
+}
+
+extern "C" int arp_anneal_chord_f32(const float* J, const float* h, const float* h0, const float* T, const float* c,
+                                    float q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1, int inst0,
+                                    float* out_e, int8_t* out_s, float* out_u, int64_t* out_fl, void* stream) {
+  if (N < 1 || N > 256) return fail(ARP_E_RANGE, "chord_f32: N = %d outside 1..256", N);
+  if (P < 1 || N % P) return fail(ARP_E_ARG, "chord_f32: P = %d does not divide N = %d", P, N);
+  if (R < 1 || S < 0 || B < 0) return fail(ARP_E_ARG, "chord_f32: R = %d, S = %d, B = %d", R, S, B);
+  if (B == 0) return ok();
+  if (!J || !h || !h0 || (S > 0 && (!T || !c)) || !out_e || !out_s) return fail(ARP_E_ARG, "chord_f32: null pointer");
+  const int waves = R < 4 ? R : 4;
+  return hip_check(arp::launch_chord_f32(J, h, h0, T, c, q, S, P, R, N, B, k0, k1, inst0, out_e, out_s, out_u,
+                                         (long long*)out_fl, waves, device_max_lds(), (hipStream_t)stream),
+                   "chord_f32 launch");
 }

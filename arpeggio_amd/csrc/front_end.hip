@@ -183,6 +183,14 @@ __global__ __launch_bounds__(256) void front_end_kernel(const float* __restrict_
     if (o.hf) o.hf[(size_t)b * D + j] = hs;
     if (o.hN)
       for (int p = 0; p < P; ++p) o.hN[(size_t)b * N + p * D + j] = h32;
+    if (o.hNf)
+      for (int p = 0; p < P; ++p) o.hNf[(size_t)b * N + p * D + j] = hs;
+  }
+  if (o.Jf) {
+    for (int x = tid; x < N * N; x += nt) {
+      const int r = x / N, c = x - r * N;
+      o.Jf[(size_t)b * N * N + x] = r == c ? 0.f : -(As[(r % D) * Ap + (c % D)] / scale);
+    }
   }
   if (o.J) {
     // J[(p,i),(p',j)] = -G_int[i][j] off the diagonal, 0 on it (chord_ref.expand)

@@ -15,6 +15,8 @@ struct FrontOut {
   int32_t* hN;     // (B, N)
   float* z;        // (B, D) LMMSE output
   float* mu;       // (B, D)
+  float* Jf;       // (B, N, N) expanded dDI couplings, float (G / scale)
+  float* hNf;      // (B, N)
 };
 
 hipError_t launch_arpeggio_fx(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R, int D,
@@ -29,5 +31,13 @@ hipError_t launch_front_end(const float* H, const float* y, const float* N0v, fl
 hipError_t launch_decode_select(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R,
                                 int P, int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat,
                                 int32_t* biterr, float* dist_out, hipStream_t st);
+
+hipError_t launch_arpeggio_f32(const float* Gt, const float* h, const float* T, int L, float q, int P, int R, int D,
+                               int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int8_t* out_s,
+                               float* out_de, long long* out_st, float* out_w, int waves, hipStream_t st);
+hipError_t launch_chord_f32(const float* J, const float* h, const float* h0, const float* T, const float* c, float q,
+                            int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1, int inst0, float* out_e,
+                            int8_t* out_s, float* out_u, long long* out_fl, int waves, size_t max_lds,
+                            hipStream_t st);
 
 }  // namespace arp

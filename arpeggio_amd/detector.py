@@ -40,10 +40,10 @@ def front_end(H, y, N0, M, K=1, Q=QSCALE, want=('xg', 'Gi', 'hi', 'scale')):
     shapes = dict(xg=((B, D), torch.int8), Gi=((B, D, D), torch.int32), hi=((B, D), torch.int32),
                   scale=((B,), torch.float32), Gf=((B, D, D), torch.float32), hf=((B, D), torch.float32),
                   J=((B, N, N), torch.int16), hN=((B, N), torch.int32), z=((B, D), torch.float32),
-                  mu=((B, D), torch.float32))
+                  mu=((B, D), torch.float32), Jf=((B, N, N), torch.float32), hNf=((B, N), torch.float32))
     out = {k: torch.empty(*shapes[k][0], dtype=shapes[k][1], device=dev) for k in want}
     n0t, n0s = _n0_arg(N0, B)
-    order = ('xg', 'Gi', 'hi', 'scale', 'Gf', 'hf', 'J', 'hN', 'z', 'mu')
+    order = ('xg', 'Gi', 'hi', 'scale', 'Gf', 'hf', 'J', 'hN', 'z', 'mu', 'Jf', 'hNf')
     L = _lib.load()
     _lib.check(L.arp_front_end(_lib.ptr(H), _lib.ptr(y), _lib.ptr(n0t), n0s, B, Mr, D, int(M), int(K), float(Q),
                                *[_lib.ptr(out.get(k)) for k in order], _lib.stream_ptr()), 'front_end')
@@ -122,7 +122,7 @@ class DetectPipeline:
         P_ = _lib.ptr
         if self.policy == 'arpeggio':
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
-                                       P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None, s),
+                                       P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None, None, None, s),
                        'front_end')
             _lib.check(L.arp_anneal_arpeggio(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D, B,
                                              P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
@@ -130,7 +130,7 @@ class DetectPipeline:
                                              P_(self.spins), P_(self.de), P_(self.st), s), 'arpeggio')
         else:
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg), None,
-                                       None, P_(self.scale), None, None, P_(self.J), P_(self.hN), None, None, s),
+                                       None, P_(self.scale), None, None, P_(self.J), P_(self.hN), None, None, None, None, s),
                        'front_end')
             _lib.check(L.arp_anneal_chord(P_(self.J), P_(self.hN), P_(self.hN), P_(self.T), P_(self.c), self.q, self.S,
                                           P, R, P * D, B, self.key & MASK, (self.key >> 32) & MASK, self.inst0, 0,

@@ -69,10 +69,11 @@ int arp_anneal_chord(const int16_t *J, const int32_t *h, const int32_t *h0, cons
  *   K: dDI radius (P = 2K copies)    Q: fixed-point scale (4096)
  * Outputs (each may be NULL): xg (B, D) int8 levels; Gi (B, D, D) int32 = rne(G / scale Q) (symmetric, so also the
  * transposed Gram the Arpeggio kernel takes); hi (B, D) int32; scale (B); Gf (B, D, D), hf (B, D) float32 G/scale,
- * h/scale; J (B, N, N) int16 and hN (B, N) int32, the expanded Chord inputs (N = 2 K D <= 256); z, mu (B, D). */
+ * h/scale; J (B, N, N) int16 and hN (B, N) int32, the expanded Chord inputs (N = 2 K D <= 256); z, mu (B, D);
+ * Jf (B, N, N), hNf (B, N) float32, the expanded inputs of the FP32 Chord anneal. */
 int arp_front_end(const float *H, const float *y, const float *N0, float N0_scalar, int B, int Mr, int D, int M, int K,
                   float Q, int8_t *xg, int32_t *Gi, int32_t *hi, float *scale, float *Gf, float *hf, int16_t *J,
-                  int32_t *hN, float *z, float *mu, void *stream);
+                  int32_t *hN, float *z, float *mu, float *Jf, float *hNf, void *stream);
 
 /* Decode and select: candidate levels from anneal spins (delta = 1: clip(x_g + sum of the P copies); delta = 0: RI
  * spins), lowest ||y - H x||^2 over the R candidates then x_g (first minimum), Gray bit errors against ltrue.
@@ -83,6 +84,21 @@ int arp_front_end(const float *H, const float *y, const float *N0, float N0_scal
 int arp_decode_select(const float *H, const float *y, const int8_t *xg, const int8_t *spins, int B, int R, int P, int D,
                       int Mr, int M, int delta, const int8_t *ltrue, int8_t *lhat, int32_t *biterr, float *dist,
                       void *stream);
+
+/* FP32 Arpeggio anneal on float32 scaled couplings (Gt = G / scale transposed, h = h / scale, T and q scaled): the
+ * float twin of arp_anneal_arpeggio, acceptance z < ((float)x 2^-32 - 1/2) 4T on the same Philox words. The
+ * reference's float first-accept solver is arpeggio/sim/arpx55.py:29-99 (rule 'r', torch generator draws).
+ *   out_de (B, R) float32 best energy change in E0 units (-h.s - s^T J s)   out_w (B, R, D) final fields or NULL */
+int arp_anneal_arpeggio_f32(const float *Gt, const float *h, const float *T, int L, float q, int P, int R, int D,
+                            int B, const uint32_t *inst, uint32_t k0, uint32_t k1, int start, int8_t *out_s,
+                            float *out_de, int64_t *out_st, float *out_w, void *stream);
+
+/* FP32 Chord anneal on float32 symmetric J (B, N, N), h, h0 (B, N), T, c (S) float32: the float twin of
+ * arp_anneal_chord with Philox draws (the reference's float Chord is mimo.chord, arpeggio/mimo/mimo.py:204-249).
+ *   out_e (B, R) best E0, out_s (B, R, N) int8, out_u (B, R, N) final fields or NULL, out_fl (B, R) flips or NULL */
+int arp_anneal_chord_f32(const float *J, const float *h, const float *h0, const float *T, const float *c, float q, int S,
+                         int P, int R, int N, int B, uint32_t k0, uint32_t k1, int inst0, float *out_e, int8_t *out_s,
+                         float *out_u, int64_t *out_fl, void *stream);
 
 #ifdef __cplusplus
 }
