@@ -85,11 +85,12 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G, c
       const int32_t Tk = T[k];
       const uint32_t A = 4u * (uint32_t)Tk;
       const int32_t Bk = q + 2 * Tk;
-      uint32_t x[CPL];
-      int32_t zc[CPL];
+      // flip iff s w + 2 G_jj + B_k < thr(x, 4 T_k)
+      int32_t thr[CPL], zc[CPL];
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        x[c] = nm == 0 ? rw[p][c][0] : nm == 1 ? rw[p][c][1] : nm == 2 ? rw[p][c][2] : rw[p][c][3];
+        const uint32_t x = nm == 0 ? rw[p][c][0] : nm == 1 ? rw[p][c][1] : nm == 2 ? rw[p][c][2] : rw[p][c][3];
+        thr[c] = accept_threshold(x, A);
         zc[c] = dj2[c] + Bk;
       }
       long long tt = 1;
@@ -100,7 +101,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G, c
           while (rem) {
             const bool s_pos = (sb[c] >> p) & 1u;
             const int32_t z = (s_pos ? w[c] : -w[c]) + zc[c];
-            const uint64_t m = ballot(accept_fx(x[c], A, z)) & rem;
+            const uint64_t m = ballot(z < thr[c]) & rem;
             if (m == 0) break;
             const int l = __builtin_ctzll(m);
             const int jf = c * ARP_WAVE + l;
@@ -125,7 +126,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G, c
         for (int c = 0; c < CPL; ++c) {
           const bool s_pos = (sb[c] >> p) & 1u;
           const int32_t z = (s_pos ? w[c] : -w[c]) + zc[c];
-          fm[c] = ballot(accept_fx(x[c], A, z)) & vmask[c];
+          fm[c] = ballot(z < thr[c]) & vmask[c];
         }
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {

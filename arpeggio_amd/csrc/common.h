@@ -10,13 +10,16 @@
 namespace arp {
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of two 32x32->64 multiplies with the Random123
-// multipliers and Weyl key increments. The key bumps are wave-uniform and land on the scalar unit.
+// multipliers and Weyl key increments. Each product is written as one 64-bit multiply so that it issues as a single
+// v_mad_u64_u32 (both halves) rather than a v_mul_hi_u32 / v_mul_lo_u32 pair. The key bumps are wave-uniform and
+// land on the scalar unit.
 __device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
                                          uint32_t k1) {
 #pragma unroll
   for (int i = 0; i < 10; ++i) {
-    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     const uint32_t x0 = hi1 ^ c1 ^ k0, x2 = hi0 ^ c3 ^ k1;
     c0 = x0;
     c1 = lo1;
@@ -45,11 +48,18 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (ARP_WAVE 
 
 // Exact form of the reference's 64-bit decision  x * A > z * 2^32  (arpeggio_fx.cu:76-79, chord_kernel.cu:114)
 // for 0 <= A < 2^31 and any int32 z: with x*A = hi*2^32 + lo, it holds iff hi > z, or hi == z and lo > 0
-// (for z < 0 both sides agree because x*A >= 0). Two 32-bit multiplies instead of a 64-bit signed compare.
+// (for z < 0 both sides agree because x*A >= 0). One 32x32->64 product instead of a 64-bit signed compare.
 __device__ __forceinline__ bool accept_fx(uint32_t x, uint32_t A, int32_t z) {
-  const int32_t hi = (int32_t)__umulhi(x, A);
-  const uint32_t lo = x * A;
-  return hi > z || (hi == z && lo != 0u);
+  const uint64_t pr = (uint64_t)x * A;
+  const int32_t hi = (int32_t)(pr >> 32);
+  return hi > z || (hi == z && (uint32_t)pr != 0u);
+}
+
+// The same test as a threshold that depends on x and A only: x*A > z*2^32  <=>  z < hi + (lo != 0). One threshold per
+// (lane, pass) turns every re-test of the first-accept scan into a single compare.
+__device__ __forceinline__ int32_t accept_threshold(uint32_t x, uint32_t A) {
+  const uint64_t pr = (uint64_t)x * A;
+  return (int32_t)(pr >> 32) + ((uint32_t)pr != 0u ? 1 : 0);
 }
 
 // The i-th P-bit word with P/2 ones in increasing numeric order (arpeggio_ref59.py:31-38), for P = 2, 4, 6.

@@ -13,7 +13,8 @@
 //      its diagonal included, on the cross-copy blocks), clamped at 1e-12.
 //   5. G_int = rne(G / scale * Q) (G symmetric by construction), h_int = rne(h / scale * Q); optionally the float
 //      scaled G and h for the FP32 annealers, and the expanded Chord inputs J (N x N int16, N = 2K D) and h (N).
-// Sizes: D <= 128 (2 Nt), Mr <= 128 (2 Nr); both are zero-padded inside LDS to multiples of 16 and 4.
+// Sizes: D <= 128 (2 Nt), Mr <= 128 (2 Nr); both are zero-padded inside LDS to multiples of 16 and 4. Systems with
+// D, Mr <= 64 take the one-wave register form below; larger ones the four-wave LDS form.
 #include "kernels.h"
 
 namespace arp {
@@ -203,13 +204,163 @@ __global__ __launch_bounds__(256) void front_end_kernel(const float* __restrict_
   }
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// One-wave form for D <= 64 and Mr <= 64 (16 and 32 users): lane j owns column j of A = G + sigma I in registers and
+// inverts it in place by Gauss-Jordan without pivoting (A is symmetric positive definite): step k broadcasts column
+// k with v_readlane and every lane applies one FMA per element, D^2 FMAs per lane in all, no barriers. A^-1 is
+// symmetric, so lane i's column is row i: z_i = sum_j (A^-1)_ij c_j and (A^-1)_ii stay in lane i. h is formed as
+// 2 (c - G x_g) from the LDS-resident Gram. LDS: H (Mp x (DP+1)) and G (DP x (DP+1)), padded rows against bank
+// conflicts.
+template <int DP>
+__global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restrict__ H, const float* __restrict__ y,
+                                                           const float* __restrict__ N0v, float N0s, int Mr, int D,
+                                                           int M, int K, float Q, FrontOut o) {
+  extern __shared__ float sm[];
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int Mp = (Mr + 3) & ~3, ld = DP + 1;
+  float* Hs = sm;             // Mp x ld
+  float* Gs = Hs + Mp * ld;   // DP x ld
+  float* ys = Gs + DP * ld;   // Mp
+  const float* Hb = H + (size_t)b * Mr * D;
+  for (int x = lane; x < Mp * DP; x += 64) {
+    const int r = x / DP, c = x - r * DP;
+    Hs[r * ld + c] = (r < Mr && c < D) ? Hb[(size_t)r * D + c] : 0.f;
+  }
+  for (int x = lane; x < Mp; x += 64) ys[x] = x < Mr ? y[(size_t)b * Mr + x] : 0.f;
+  __syncthreads();
+  // Gram on the matrix cores (upper tiles, mirrored)
+  {
+    constexpr int nT = DP / 16;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int I = 0; I < nT; ++I)
+#pragma unroll
+      for (int J = I; J < nT; ++J) {
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int kk = 0; kk < Mp; kk += 4) {
+          const float a = Hs[(kk + lk) * ld + 16 * I + li];
+          const float bb = Hs[(kk + lk) * ld + 16 * J + li];
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          Gs[(16 * I + 4 * lk + v) * ld + 16 * J + li] = acc[v];
+          Gs[(16 * J + li) * ld + 16 * I + 4 * lk + v] = acc[v];
+        }
+      }
+  }
+  __syncthreads();
+  const float el2 = M == 4 ? 1.f : 5.f;
+  const float n0 = N0v ? N0v[b] : N0s;
+  const float sigma = (n0 / 2.f) / el2;
+  const bool valid = lane < D;
+  // c_j = (H^T y)_j and column j of A (identity on the padding)
+  float c = 0.f;
+  for (int k = 0; k < Mr; ++k) c += Hs[k * ld + lane] * ys[k];
+  float a[DP];
+#pragma unroll
+  for (int i = 0; i < DP; ++i) {
+    const float g = lane < DP ? Gs[i * ld + lane] : 0.f;
+    a[i] = valid ? (i == lane ? g + sigma : (i < D ? g : 0.f)) : (i == lane ? 1.f : 0.f);
+  }
+  // Gauss-Jordan, in place: after step k, column k holds -A_ik / A_kk (i != k) and 1 / A_kk; other columns are
+  // reduced by the scaled pivot row. Lane k takes rk = 1 + p so that the shared update gives its -f p.
+#pragma unroll
+  for (int k = 0; k < DP; ++k) {
+    const float akk = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a[k]), k));
+    const float p = 1.f / akk;
+    const bool is_k = lane == k;
+    const float rk = is_k ? 1.f + p : a[k] * p;
+#pragma unroll
+    for (int i = 0; i < DP; ++i) {
+      if (i == k) continue;
+      const float f = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a[i]), k));
+      a[i] = __builtin_fmaf(-f, rk, a[i]);
+    }
+    a[k] = is_k ? p : rk;
+  }
+  // z_i = sum_j (A^-1)_ij c_j, d_i = (A^-1)_ii, mu_i = 1 - sigma d_i, x_g = nearest level of z/mu
+  float z = 0.f, d = 0.f;
+#pragma unroll
+  for (int j = 0; j < DP; ++j) {
+    const float cj = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c), j));
+    z = __builtin_fmaf(a[j], cj, z);
+    d = j == lane ? a[j] : d;
+  }
+  const float mu = 1.f - sigma * d;
+  const float v = z / mu;
+  const float lmax = M == 4 ? 1.f : 3.f;
+  float l = fminf(fmaxf(2.f * ceilf(v * 0.5f) - 1.f, -lmax), lmax);
+  if (v != v) l = -lmax;
+  if (!valid) l = 0.f;
+  if (valid) {
+    if (o.z) o.z[(size_t)b * D + lane] = z;
+    if (o.mu) o.mu[(size_t)b * D + lane] = mu;
+    if (o.xg) o.xg[(size_t)b * D + lane] = (int8_t)l;
+  }
+  // h_j = 2 (c_j - (G x_g)_j); scale = max(max|h|, max|G|)
+  float gx = 0.f, gm = 0.f;
+  for (int i = 0; i < D; ++i) {
+    const float gij = Gs[i * ld + lane];
+    const float li_ = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, l), i));
+    gx = __builtin_fmaf(gij, li_, gx);
+    gm = fmaxf(gm, fabsf(gij));
+  }
+  const float hval = valid ? 2.f * (c - gx) : 0.f;
+  float m = valid ? fmaxf(gm, fabsf(hval)) : 0.f;
+  for (int off = 32; off; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  const float scale = fmaxf(m, 1e-12f);
+  if (lane == 0 && o.scale) o.scale[b] = scale;
+  const int P = 2 * K, N = P * D;
+  if (valid) {
+    const float hs = hval / scale;
+    const int32_t h32 = (int32_t)rintf(hs * Q);
+    if (o.hi) o.hi[(size_t)b * D + lane] = h32;
+    if (o.hf) o.hf[(size_t)b * D + lane] = hs;
+    for (int p = 0; p < P; ++p) {
+      if (o.hN) o.hN[(size_t)b * N + p * D + lane] = h32;
+      if (o.hNf) o.hNf[(size_t)b * N + p * D + lane] = hs;
+    }
+    for (int i = 0; i < D; ++i) {
+      const float gs = Gs[i * ld + lane] / scale;
+      const int32_t gi = (int32_t)rintf(gs * Q);
+      if (o.Gi) o.Gi[(size_t)b * D * D + i * D + lane] = gi;
+      if (o.Gf) o.Gf[(size_t)b * D * D + i * D + lane] = gs;
+      if (o.J || o.Jf)
+        for (int pr = 0; pr < P; ++pr)
+          for (int pc = 0; pc < P; ++pc) {
+            const size_t x = (size_t)b * N * N + (size_t)(pr * D + i) * N + pc * D + lane;
+            const bool diag = pr == pc && i == lane;
+            if (o.J) o.J[x] = diag ? (int16_t)0 : (int16_t)(-gi);
+            if (o.Jf) o.Jf[x] = diag ? 0.f : -gs;
+          }
+    }
+  }
+}
+
 size_t front_end_lds_bytes(int Mr, int D) {
   const int Dp = (D + 15) & ~15, Mp = (Mr + 3) & ~3;
   return sizeof(float) * ((size_t)Mp * Dp + (size_t)Dp * (Dp + 1) + Mp + 6 * Dp + 8);
 }
 
+template <int DP>
+static void launch_w64(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M, int K,
+                       float Q, const FrontOut& o, hipStream_t st) {
+  const int Mp = (Mr + 3) & ~3;
+  const size_t lds = sizeof(float) * ((size_t)(Mp + DP) * (DP + 1) + Mp);
+  hipLaunchKernelGGL((front_end_w64_kernel<DP>), dim3(B), dim3(64), lds, st, H, y, N0v, N0s, Mr, D, M, K, Q, o);
+}
+
 hipError_t launch_front_end(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M,
                             int K, float Q, const FrontOut& o, hipStream_t st) {
+  if (D <= 64 && Mr <= 64) {
+    const int DP = (D + 15) & ~15;
+    if (DP == 16) launch_w64<16>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);
+    else if (DP == 32) launch_w64<32>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);
+    else if (DP == 48) launch_w64<48>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);
+    else launch_w64<64>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);
+    return hipGetLastError();
+  }
   const size_t lds = front_end_lds_bytes(Mr, D);
   if (lds > 65536) hipFuncSetAttribute((const void*)front_end_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(front_end_kernel, dim3(B), dim3(256), lds, st, H, y, N0v, N0s, Mr, D, M, K, Q, o);

@@ -31,7 +31,7 @@ def parse():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--frames', type=int, default=65536)
-    ap.add_argument('--snr', type=float, default=18.0)
+    ap.add_argument('--snr', type=float, default=24.0)
     ap.add_argument('--copies', type=int, default=16)
     ap.add_argument('--passes', type=int, default=64)
     ap.add_argument('--seed', type=int, default=1)
