@@ -207,10 +207,10 @@ __global__ __launch_bounds__(256) void front_end_kernel(const float* __restrict_
 // ---------------------------------------------------------------------------------------------------------------
 // One-wave form for D <= 64 and Mr <= 64 (16 and 32 users): lane j owns column j of A = G + sigma I in registers and
 // inverts it in place by Gauss-Jordan without pivoting (A is symmetric positive definite): step k broadcasts column
-// k with v_readlane and every lane applies one FMA per element, D^2 FMAs per lane in all, no barriers. A^-1 is
-// symmetric, so lane i's column is row i: z_i = sum_j (A^-1)_ij c_j and (A^-1)_ii stay in lane i. h is formed as
-// 2 (c - G x_g) from the LDS-resident Gram. LDS: H (Mp x (DP+1)) and G (DP x (DP+1)), padded rows against bank
-// conflicts.
+// k through LDS and every lane applies one FMA per element, D^2 FMAs per lane in all. A^-1 is symmetric, so lane
+// i's column is row i: z_i = sum_j (A^-1)_ij c_j and (A^-1)_ii stay in lane i. h is formed as 2 (c - G x_g) from
+// the LDS-resident Gram. LDS holds H (Mp x (DP+1)) until the Gram tiles and c are formed, then G (DP x (DP+1)) in
+// the same place: 17 KiB per frame at D = 64, rows padded against bank conflicts.
 template <int DP>
 __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restrict__ H, const float* __restrict__ y,
                                                            const float* __restrict__ N0v, float N0s, int Mr, int D,
@@ -218,45 +218,60 @@ __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restri
   extern __shared__ float sm[];
   const int b = blockIdx.x, lane = threadIdx.x;
   const int Mp = (Mr + 3) & ~3, ld = DP + 1;
-  float* Hs = sm;             // Mp x ld
-  float* Gs = Hs + Mp * ld;   // DP x ld
-  float* ys = Gs + DP * ld;   // Mp
+  const int rows = Mp > DP ? Mp : DP;
+  float* Hs = sm;             // Mp x ld, later G (DP x ld) in the same place
+  float* Gs = sm;
+  float* ys = sm + rows * ld; // Mp
+  float* colb = ys + Mp;      // 64 words: broadcast buffer
   const float* Hb = H + (size_t)b * Mr * D;
   for (int x = lane; x < Mp * DP; x += 64) {
-    const int r = x / DP, c = x - r * DP;
-    Hs[r * ld + c] = (r < Mr && c < D) ? Hb[(size_t)r * D + c] : 0.f;
+    const int r = x / DP, cc = x - r * DP;
+    Hs[r * ld + cc] = (r < Mr && cc < D) ? Hb[(size_t)r * D + cc] : 0.f;
   }
   for (int x = lane; x < Mp; x += 64) ys[x] = x < Mr ? y[(size_t)b * Mr + x] : 0.f;
   __syncthreads();
-  // Gram on the matrix cores (upper tiles, mirrored)
+  // Gram on the matrix cores: the upper 16x16 tiles stay in the accumulators until H is no longer needed
+  constexpr int nT = DP / 16, nU = nT * (nT + 1) / 2;
+  floatx4 acc[nU];
   {
-    constexpr int nT = DP / 16;
     const int li = lane & 15, lk = lane >> 4;
+    int u = 0;
 #pragma unroll
     for (int I = 0; I < nT; ++I)
 #pragma unroll
-      for (int J = I; J < nT; ++J) {
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int J = I; J < nT; ++J, ++u) {
+        floatx4 t = {0.f, 0.f, 0.f, 0.f};
         for (int kk = 0; kk < Mp; kk += 4) {
-          const float a = Hs[(kk + lk) * ld + 16 * I + li];
-          const float bb = Hs[(kk + lk) * ld + 16 * J + li];
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, acc, 0, 0, 0);
+          const float a0 = Hs[(kk + lk) * ld + 16 * I + li];
+          const float b0 = Hs[(kk + lk) * ld + 16 * J + li];
+          t = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, t, 0, 0, 0);
         }
+        acc[u] = t;
+      }
+  }
+  // c_j = (H^T y)_j
+  float c = 0.f;
+  for (int k = 0; k < Mr; ++k) c += Hs[k * ld + lane] * ys[k];
+  __syncthreads();
+  {
+    const int li = lane & 15, lk = lane >> 4;
+    int u = 0;
+#pragma unroll
+    for (int I = 0; I < nT; ++I)
+#pragma unroll
+      for (int J = I; J < nT; ++J, ++u)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          Gs[(16 * I + 4 * lk + v) * ld + 16 * J + li] = acc[v];
-          Gs[(16 * J + li) * ld + 16 * I + 4 * lk + v] = acc[v];
+          Gs[(16 * I + 4 * lk + v) * ld + 16 * J + li] = acc[u][v];
+          Gs[(16 * J + li) * ld + 16 * I + 4 * lk + v] = acc[u][v];
         }
-      }
   }
   __syncthreads();
   const float el2 = M == 4 ? 1.f : 5.f;
   const float n0 = N0v ? N0v[b] : N0s;
   const float sigma = (n0 / 2.f) / el2;
   const bool valid = lane < D;
-  // c_j = (H^T y)_j and column j of A (identity on the padding)
-  float c = 0.f;
-  for (int k = 0; k < Mr; ++k) c += Hs[k * ld + lane] * ys[k];
+  // column j of A (identity on the padding)
   float a[DP];
 #pragma unroll
   for (int i = 0; i < DP; ++i) {
@@ -264,29 +279,44 @@ __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restri
     a[i] = valid ? (i == lane ? g + sigma : (i < D ? g : 0.f)) : (i == lane ? 1.f : 0.f);
   }
   // Gauss-Jordan, in place: after step k, column k holds -A_ik / A_kk (i != k) and 1 / A_kk; other columns are
-  // reduced by the scaled pivot row. Lane k takes rk = 1 + p so that the shared update gives its -f p.
-#pragma unroll
+  // reduced by the scaled pivot row. Lane k takes rk = 1 + p so that the shared update gives its -f p. Column k is
+  // broadcast through LDS (lane k stores it, every lane reads it back at one address per word), so the update runs
+  // on vector registers; both loops are fully unrolled so that a[k] and a[i] are fixed registers.
+#pragma clang loop unroll(full)
   for (int k = 0; k < DP; ++k) {
-    const float akk = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a[k]), k));
-    const float p = 1.f / akk;
+    if (lane == k) {
+#pragma unroll
+      for (int i = 0; i < DP; i += 4)
+        *reinterpret_cast<floatx4*>(colb + i) = floatx4{a[i], a[i + 1], a[i + 2], a[i + 3]};
+    }
+    __syncthreads();
+    const float p = 1.f / colb[k];
     const bool is_k = lane == k;
     const float rk = is_k ? 1.f + p : a[k] * p;
+#pragma clang loop unroll(full)
+    for (int i = 0; i < DP; i += 4) {
+      const floatx4 f = *reinterpret_cast<const floatx4*>(colb + i);
 #pragma unroll
-    for (int i = 0; i < DP; ++i) {
-      if (i == k) continue;
-      const float f = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a[i]), k));
-      a[i] = __builtin_fmaf(-f, rk, a[i]);
+      for (int v = 0; v < 4; ++v)
+        if (i + v != k) a[i + v] = __builtin_fmaf(-f[v], rk, a[i + v]);
     }
     a[k] = is_k ? p : rk;
+    __syncthreads();
   }
-  // z_i = sum_j (A^-1)_ij c_j, d_i = (A^-1)_ii, mu_i = 1 - sigma d_i, x_g = nearest level of z/mu
+  // z_i = sum_j (A^-1)_ij c_j, d_i = (A^-1)_ii; c and, below, x_g are broadcast through LDS the same way
+  colb[lane] = c;
+  __syncthreads();
   float z = 0.f, d = 0.f;
 #pragma unroll
-  for (int j = 0; j < DP; ++j) {
-    const float cj = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c), j));
-    z = __builtin_fmaf(a[j], cj, z);
-    d = j == lane ? a[j] : d;
+  for (int j = 0; j < DP; j += 4) {
+    const floatx4 cj = *reinterpret_cast<const floatx4*>(colb + j);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      z = __builtin_fmaf(a[j + v], cj[v], z);
+      d = j + v == lane ? a[j + v] : d;
+    }
   }
+  __syncthreads();
   const float mu = 1.f - sigma * d;
   const float v = z / mu;
   const float lmax = M == 4 ? 1.f : 3.f;
@@ -299,11 +329,12 @@ __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restri
     if (o.xg) o.xg[(size_t)b * D + lane] = (int8_t)l;
   }
   // h_j = 2 (c_j - (G x_g)_j); scale = max(max|h|, max|G|)
+  colb[lane] = l;
+  __syncthreads();
   float gx = 0.f, gm = 0.f;
   for (int i = 0; i < D; ++i) {
     const float gij = Gs[i * ld + lane];
-    const float li_ = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, l), i));
-    gx = __builtin_fmaf(gij, li_, gx);
+    gx = __builtin_fmaf(gij, colb[i], gx);
     gm = fmaxf(gm, fabsf(gij));
   }
   const float hval = valid ? 2.f * (c - gx) : 0.f;
@@ -347,7 +378,7 @@ template <int DP>
 static void launch_w64(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M, int K,
                        float Q, const FrontOut& o, hipStream_t st) {
   const int Mp = (Mr + 3) & ~3;
-  const size_t lds = sizeof(float) * ((size_t)(Mp + DP) * (DP + 1) + Mp);
+  const size_t lds = sizeof(float) * ((size_t)(Mp > DP ? Mp : DP) * (DP + 1) + Mp + 64);
   hipLaunchKernelGGL((front_end_w64_kernel<DP>), dim3(B), dim3(64), lds, st, H, y, N0v, N0s, Mr, D, M, K, Q, o);
 }
 
