@@ -264,3 +264,21 @@ def test_boot_ratio_mcnemar(golden, orc):
         a, b = z['stats/%d/a' % i], z['stats/%d/b' % i]
         assert list(orc.boot_ratio(a, b, B=2000)) == r['boot']
         assert list(orc.mcnemar(a, b)) == r['mcnemar']
+
+
+def test_draws_match_reference(golden):
+    """frames.draw (the product's frame source) reproduces the reference's arpx.draw bytes: H, y, true levels."""
+    import torch
+    from arpeggio_amd import frames
+    z, idx = golden
+    for tag, f in idx['front'].items():
+        if tag == 'tofixed':
+            continue
+        p = 'front/%s/' % tag
+        assert frames.draw_seed(f['seed'], f['snr'], f['bi']) == f['draw_seed']
+        b = frames.draw(f['seed'], f['snr'], f['bi'], z[p + 'H'].shape[0], f['nt'], f['M'], dev='cpu')
+        assert np.array_equal(b['H'].numpy(), z[p + 'H']) and np.array_equal(b['y'].numpy(), z[p + 'y'])
+        assert np.array_equal(b['l'].numpy(), z[p + 'l']) and b['N0'] == f['N0']
+        lt = torch.as_tensor(z[p + 'l'])
+        bits = frames.levels_to_bits(lt, f['M'])
+        assert bits.shape[-1] == lt.shape[-1] * (2 if f['M'] == 16 else 1)
