@@ -27,6 +27,8 @@ def _range_check(G, h, P, T, q):
     dmax = int(torch.diagonal(Ga, dim1=-2, dim2=-1).max())
     if wmax + 2 * dmax + abs(int(q)) + 2 * Tmax >= INT31:
         raise ValueError('arpeggio: |h| + 2P sum|G| = %d exceeds the int32 field range' % wmax)
+    if 6 * Tmax + 2 * dmax + abs(int(q)) >= INT31:
+        raise ValueError('arpeggio: 4T + 2|G_jj| + |q| + 2T exceeds the int32 decision range')
 
 
 def solve_device(G, h, P, R, T, q, inst, key, start, mode='arpeggio', transposed=False):

@@ -9,15 +9,24 @@
 // parallel against their fields and the sequential index-order scan becomes
 //     mask = ballot(accepted & j >= ptr);  j* = ffs(mask);  commit j*;  ptr = j* + 1;  re-test
 // The commit reads one Gram column (D consecutive LDS words: bank-conflict free) and updates every lane's field.
-// de, best, flips and tests are wave-uniform and live in scalar registers; spins are one bit per copy per lane.
+// de, best, flips and the spin masks are wave-uniform and live in scalar registers.
 // Each (lane, copy) Philox block gives the words of 4 consecutive sweeps, so a block is computed every 4th sweep of
 // a copy instead of once per test.
 #include "kernels.h"
 
 namespace arp {
 
+// One anneal per wave. G4 is the frame's Gram column block in LDS pre-scaled by 4 (G4[j*D + i] = 4 G_ij), so a flip of
+// lane j with old spin s adds s*G4[j][i] to every w_i and 4 (s w_j + 2 G_jj) to de.
+// Spins: Sm[p][c] is a wave-uniform mask (bit l: spin of copy p at coordinate c*64 + l is +1), so committing a flip
+// and saving the best state are scalar operations.
+// Test: lane j of copy p flips iff q_j = s w_j + 2 G_jj < thr(x, 4 T_k) - B_k, with q_j formed on the vector unit
+// (two ops) each time the scan re-tests; the committed lane's q_j is also its energy step (de += 4 q_j), so a flip
+// reads one lane value. The scalar unit issues one instruction per SIMD every 4 cycles, the same rate as a wave64
+// vector instruction, so the loop keeps per-flip scalar work to the mask and counter updates.
+// tests (the reference's comparator passes) = L + flips - flips on the last lane (arpeggio_ref59.py:60-71).
 template <int P, int CPL>
-__device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G, const int32_t* __restrict__ hb,
+__device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb,
                                                 const int32_t* __restrict__ T, int L, int q, int D, uint32_t r,
                                                 uint32_t inst, uint32_t k0, uint32_t k1, int start, int mode,
                                                 int8_t* __restrict__ out_s, long long* __restrict__ out_de,
@@ -25,136 +34,138 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G, c
   const int lane = lane_id();
   bool valid[CPL];
   int32_t w[CPL], dj2[CPL];
-  uint32_t sb[CPL], bsb[CPL];
   uint64_t vmask[CPL];
+  uint64_t Sm[P][CPL], Bm[P][CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     const int j = c * ARP_WAVE + lane;
     valid[c] = j < D;
     const int nv = D - c * ARP_WAVE;
     vmask[c] = nv >= ARP_WAVE ? ~0ull : (nv <= 0 ? 0ull : ((1ull << nv) - 1ull));
-    dj2[c] = valid[c] ? 2 * G[j * D + j] : 0;
-    uint32_t bits = 0;
-    if (valid[c]) {
-      if (start == 0) {
+    dj2[c] = valid[c] ? G4[j * D + j] >> 1 : 0;
+    if (start == 0) {
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-          bits |= (philox_w0(0xFFFFFFFFu, (uint32_t)(p * D + j), r, inst, k0, k1) >= 0x80000000u ? 1u : 0u) << p;
-      } else {
-        const uint32_t x = philox_w0(0xFFFFFFFFu, (uint32_t)j, r, inst, k0, k1);
-        bits = bal_pattern(P, (uint32_t)(((uint64_t)x * bal_count(P)) >> 32));
-      }
+      for (int p = 0; p < P; ++p)
+        Sm[p][c] = ballot(valid[c] &&
+                          philox_w0(0xFFFFFFFFu, (uint32_t)(p * D + j), r, inst, k0, k1) >= 0x80000000u);
+    } else {
+      const uint32_t x = valid[c] ? philox_w0(0xFFFFFFFFu, (uint32_t)j, r, inst, k0, k1) : 0u;
+      const uint32_t bits = bal_pattern(P, (uint32_t)(((uint64_t)x * bal_count(P)) >> 32));
+#pragma unroll
+      for (int p = 0; p < P; ++p) Sm[p][c] = ballot(valid[c] && ((bits >> p) & 1u));
     }
-    sb[c] = bits;
-    bsb[c] = bits;
+#pragma unroll
+    for (int p = 0; p < P; ++p) Bm[p][c] = Sm[p][c];
     w[c] = valid[c] ? hb[j] : 0;
   }
-  // w_i = h_i - 2 sum_j G_ij m_j, m_j = sum_p s_pj = 2 popcount - P
+  // w_i = h_i - 2 sum_j G_ij m_j = h_i - sum_j G4_ij (m_j / 2), m_j = sum_p s_pj (even: P is even)
 #pragma unroll
   for (int c2 = 0; c2 < CPL; ++c2) {
-    const int mj_lane = valid[c2] ? 2 * __builtin_popcount(sb[c2]) - P : 0;
     const int nj = D - c2 * ARP_WAVE < ARP_WAVE ? D - c2 * ARP_WAVE : ARP_WAVE;
     for (int l = 0; l < nj; ++l) {
-      const int mj = readlane(mj_lane, l);
-      if (mj != 0) {
-        const int32_t* col = G + (c2 * ARP_WAVE + l) * D;
+      int half_m = 0;
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) w[c] -= 2 * mj * col[c * ARP_WAVE + lane];
+      for (int p = 0; p < P; ++p) half_m += (int)((Sm[p][c2] >> l) & 1ull);
+      half_m -= P / 2;
+      if (half_m != 0) {
+        const int32_t* col = G4 + (c2 * ARP_WAVE + l) * D;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) w[c] -= half_m * col[c * ARP_WAVE + lane];
       }
     }
   }
-  long long de = 0, best = 1LL << 62, flips = 0, tests = 0;
+  long long de4 = 0, best4 = 1LL << 60;   // de / 4
+  int flips = 0, last = 0;
   uint32_t rw[P][CPL][4];
-  int n = 0;
-  for (int k = 0; k < L; ++n) {
-    const int nm = n & 3;
+  for (int kb = 0; kb < L; kb += 4 * P) {
+    const int n4 = kb / (4 * P);
+    int32_t Tb[4 * P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-      if (k >= L) break;
-      if (nm == 0) {
+    for (int i = 0; i < 4 * P; ++i) Tb[i] = kb + i < L ? T[kb + i] : 0;
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          uint32_t c0 = (uint32_t)(n >> 2), c1 = (uint32_t)(p * D + c * ARP_WAVE + lane), c2 = r, c3 = inst;
-          philox10(c0, c1, c2, c3, k0, k1);
-          rw[p][c][0] = c0;
-          rw[p][c][1] = c1;
-          rw[p][c][2] = c2;
-          rw[p][c][3] = c3;
-        }
-      }
-      const int32_t Tk = T[k];
-      const uint32_t A = 4u * (uint32_t)Tk;
-      const int32_t Bk = q + 2 * Tk;
-      // flip iff s w + 2 G_jj + B_k < thr(x, 4 T_k)
-      int32_t thr[CPL], zc[CPL];
+    for (int nm = 0; nm < 4; ++nm) {
 #pragma unroll
-      for (int c = 0; c < CPL; ++c) {
-        const uint32_t x = nm == 0 ? rw[p][c][0] : nm == 1 ? rw[p][c][1] : nm == 2 ? rw[p][c][2] : rw[p][c][3];
-        thr[c] = accept_threshold(x, A);
-        zc[c] = dj2[c] + Bk;
-      }
-      long long tt = 1;
-      if (mode == 0) {
+      for (int p = 0; p < P; ++p) {
+        const int k = kb + nm * P + p;
+        if (k >= L) break;
+        if (nm == 0) {
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          uint64_t rem = vmask[c];
-          while (rem) {
-            const bool s_pos = (sb[c] >> p) & 1u;
-            const int32_t z = (s_pos ? w[c] : -w[c]) + zc[c];
-            const uint64_t m = ballot(z < thr[c]) & rem;
-            if (m == 0) break;
-            const int l = __builtin_ctzll(m);
-            const int jf = c * ARP_WAVE + l;
-            const int32_t wj = readlane(w[c], l);
-            const int32_t djj = readlane(dj2[c], l);
-            const uint32_t so_pos = (readlane_u(sb[c], l) >> p) & 1u;
-            const int32_t so4 = so_pos ? 4 : -4;
-            de += (long long)so4 * wj + 4LL * djj;
-            const int32_t* col = G + jf * D;
-#pragma unroll
-            for (int cc = 0; cc < CPL; ++cc) w[cc] += so4 * col[cc * ARP_WAVE + lane];
-            if (lane == l) sb[c] ^= 1u << p;
-            ++flips;
-            if (jf < D - 1) ++tt;
-            rem &= (l == ARP_WAVE - 1) ? 0ull : (~0ull << (l + 1));
+          for (int c = 0; c < CPL; ++c) {
+            uint32_t c0 = (uint32_t)n4, c1 = (uint32_t)(p * D + c * ARP_WAVE + lane), c2 = r, c3 = inst;
+            philox10(c0, c1, c2, c3, k0, k1);
+            rw[p][c][0] = c0;
+            rw[p][c][1] = c1;
+            rw[p][c][2] = c2;
+            rw[p][c][3] = c3;
           }
         }
-      } else {
-        // 'par': decide every lane of the pass on one snapshot, then apply the flips in index order
-        uint64_t fm[CPL];
+        const int32_t Tk = Tb[nm * P + p];
+        const uint32_t A = 4u * (uint32_t)Tk;
+        const int32_t Bk = q + 2 * Tk;
+        int32_t tq[CPL];
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          const bool s_pos = (sb[c] >> p) & 1u;
-          const int32_t z = (s_pos ? w[c] : -w[c]) + zc[c];
-          fm[c] = ballot(z < thr[c]) & vmask[c];
-        }
+        for (int c = 0; c < CPL; ++c) tq[c] = accept_threshold(rw[p][c][nm], A) - Bk;
+        if (mode == 0) {
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          uint64_t m = fm[c];
-          while (m) {
-            const int l = __builtin_ctzll(m);
-            m &= m - 1;
-            const int jf = c * ARP_WAVE + l;
-            const int32_t wj = readlane(w[c], l);
-            const int32_t djj = readlane(dj2[c], l);
-            const uint32_t so_pos = (readlane_u(sb[c], l) >> p) & 1u;
-            const int32_t so4 = so_pos ? 4 : -4;
-            de += (long long)so4 * wj + 4LL * djj;
-            const int32_t* col = G + jf * D;
+          for (int c = 0; c < CPL; ++c) {
+            uint64_t rem = vmask[c];
+            while (true) {
+              const int32_t qv = (((Sm[p][c] >> lane) & 1ull) ? w[c] : -w[c]) + dj2[c];
+              const uint64_t m = ballot(qv < tq[c]) & rem;
+              if (m == 0) break;
+              const int l = __builtin_ctzll(m);
+              de4 += readlane(qv, l);
+              const int32_t* col = G4 + (c * ARP_WAVE + l) * D;
+              if ((Sm[p][c] >> l) & 1ull) {
 #pragma unroll
-            for (int cc = 0; cc < CPL; ++cc) w[cc] += so4 * col[cc * ARP_WAVE + lane];
-            if (lane == l) sb[c] ^= 1u << p;
-            ++flips;
+                for (int cc = 0; cc < CPL; ++cc) w[cc] += col[cc * ARP_WAVE + lane];
+              } else {
+#pragma unroll
+                for (int cc = 0; cc < CPL; ++cc) w[cc] -= col[cc * ARP_WAVE + lane];
+              }
+              Sm[p][c] ^= 1ull << l;
+              ++flips;
+              last += c * ARP_WAVE + l == D - 1;
+              rem &= (~1ull) << l;
+            }
+          }
+        } else {
+          // 'par': every lane of the pass decided on one snapshot, the flips then applied in index order
+          uint64_t fm[CPL];
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            const int32_t qv = (((Sm[p][c] >> lane) & 1ull) ? w[c] : -w[c]) + dj2[c];
+            fm[c] = ballot(qv < tq[c]) & vmask[c];
+          }
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            uint64_t m = fm[c];
+            while (m) {
+              const int l = __builtin_ctzll(m);
+              m &= m - 1;
+              const int32_t qv = (((Sm[p][c] >> lane) & 1ull) ? w[c] : -w[c]) + dj2[c];
+              de4 += readlane(qv, l);
+              const int32_t* col = G4 + (c * ARP_WAVE + l) * D;
+              if ((Sm[p][c] >> l) & 1ull) {
+#pragma unroll
+                for (int cc = 0; cc < CPL; ++cc) w[cc] += col[cc * ARP_WAVE + lane];
+              } else {
+#pragma unroll
+                for (int cc = 0; cc < CPL; ++cc) w[cc] -= col[cc * ARP_WAVE + lane];
+              }
+              Sm[p][c] ^= 1ull << l;
+              ++flips;
+            }
           }
         }
-      }
-      tests += tt;
-      if (de < best) {
-        best = de;
+        if (de4 - best4 < 0) {
+          best4 = de4;
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) bsb[c] = sb[c];
+          for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) Bm[pp][c] = Sm[pp][c];
+        }
       }
-      ++k;
     }
   }
 #pragma unroll
@@ -162,18 +173,18 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G, c
     const int j = c * ARP_WAVE + lane;
     if (valid[c]) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) out_s[t * (size_t)(P * D) + p * D + j] = ((bsb[c] >> p) & 1u) ? 1 : -1;
+      for (int p = 0; p < P; ++p) out_s[t * (size_t)(P * D) + p * D + j] = ((Bm[p][c] >> lane) & 1ull) ? 1 : -1;
     }
   }
   if (lane == 0) {
-    out_de[t] = best;
+    out_de[t] = 4 * best4;
     out_st[2 * t] = flips;
-    out_st[2 * t + 1] = tests;
+    out_st[2 * t + 1] = mode == 0 ? (long long)L + flips - last : (long long)L;
   }
 }
 
-// grid.x = B frames, block = 64 W threads. LDS: D*D int32 (+ one wave of padding so that the lanes past D of the
-// last column stay inside the allocation).
+// grid.x = B frames, block = 64 W threads. LDS: 4 Gt, D*D int32 (+ one wave of padding so that the lanes past D of
+// the last column stay inside the allocation).
 template <int P, int CPL>
 __global__ __launch_bounds__(256) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt, const int32_t* __restrict__ h,
                                                           const int32_t* __restrict__ T, int L, int q, int R, int D,
@@ -185,7 +196,7 @@ __global__ __launch_bounds__(256) void arpeggio_fx_kernel(const int32_t* __restr
   const int b = blockIdx.x;
   const int nG = D * D;
   const int32_t* g = Gt + (size_t)b * nG;
-  for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? g[x] : 0;
+  for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? 4 * g[x] : 0;
   __syncthreads();
   const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
   const uint32_t in = inst[b];
