@@ -10,6 +10,7 @@ LIB_PATH = os.path.join(HERE, 'libarpeggio_hip.so')
 
 _P = ctypes.c_void_p
 _i32, _u32, _u64 = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64
+_l, _f = ctypes.c_long, ctypes.c_float
 
 # name -> argtypes; every entry point returns int (0 = success)
 SIGNATURES = {
@@ -19,7 +20,16 @@ SIGNATURES = {
     'arp_anneal_chord': [_P, _P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _i32, _P, _P, _P,
                          _P],
     'arp_front_end': [_P, _P, _P, ctypes.c_float, _i32, _i32, _i32, _i32, _i32, ctypes.c_float, _P, _P, _P, _P, _P, _P,
-                      _P, _P, _P, _P, _P, _P, _P],
+                      _P, _P, _P, _P, _P, _P, _P, _i32, _P],
+    'arp_anneal_arpeggio_ws': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _P, _i32, _P, _P, _P, _P,
+                               _P],
+    'arp_mlp_gemm': [_P, _l, _l, _P, _l, _l, _i32, _i32, _i32, _P, _i32, _P, _P, _P, _l, _P],
+    'arp_mlp_gelu_bwd': [_P, _P, _l, _P, _P, _P],
+    'arp_mlp_bce': [_P, _l, _P, _i32, _i32, _P, _P, _P, _P],
+    'arp_mlp_colsum': [_P, _l, _i32, _i32, _P, _P],
+    'arp_mlp_adam': [_P, _P, _P, _P, _l, _f, _f, _f, _f, _i32, _P, _P],
+    'arp_cast_bf16': [_P, _l, _P, _P],
+    'arp_start_thresholds': [_P, _l, _i32, _i32, _f, _P, _P],
     'arp_anneal_arpeggio_f32': [_P, _P, _P, _i32, ctypes.c_float, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _P, _P,
                                 _P, _P, _P],
     'arp_anneal_chord_f32': [_P, _P, _P, _P, _P, ctypes.c_float, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _P, _P,

@@ -25,40 +25,13 @@ namespace arp {
 // reads one lane value. The scalar unit issues one instruction per SIMD every 4 cycles, the same rate as a wave64
 // vector instruction, so the loop keeps per-flip scalar work to the mask and counter updates.
 // tests (the reference's comparator passes) = L + flips - flips on the last lane (arpeggio_ref59.py:60-71).
+// w_i = h_i - 2 sum_j G_ij m_j = h_i - sum_j G4_ij (m_j / 2) for the state in Sm (m_j = sum_p s_pj, even: P is even)
 template <int P, int CPL>
-__device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb,
-                                                const int32_t* __restrict__ T, int L, int q, int D, uint32_t r,
-                                                uint32_t inst, uint32_t k0, uint32_t k1, int start, int mode,
-                                                int8_t* __restrict__ out_s, long long* __restrict__ out_de,
-                                                long long* __restrict__ out_st, size_t t) {
+__device__ __forceinline__ void init_fields(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb, int D,
+                                            const bool* valid, uint64_t (&Sm)[P][CPL], int32_t* w) {
   const int lane = lane_id();
-  bool valid[CPL];
-  int32_t w[CPL], dj2[CPL];
-  uint64_t vmask[CPL];
-  uint64_t Sm[P][CPL], Bm[P][CPL];
 #pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    const int j = c * ARP_WAVE + lane;
-    valid[c] = j < D;
-    const int nv = D - c * ARP_WAVE;
-    vmask[c] = nv >= ARP_WAVE ? ~0ull : (nv <= 0 ? 0ull : ((1ull << nv) - 1ull));
-    dj2[c] = valid[c] ? G4[j * D + j] >> 1 : 0;
-    if (start == 0) {
-#pragma unroll
-      for (int p = 0; p < P; ++p)
-        Sm[p][c] = ballot(valid[c] &&
-                          philox_w0(0xFFFFFFFFu, (uint32_t)(p * D + j), r, inst, k0, k1) >= 0x80000000u);
-    } else {
-      const uint32_t x = valid[c] ? philox_w0(0xFFFFFFFFu, (uint32_t)j, r, inst, k0, k1) : 0u;
-      const uint32_t bits = bal_pattern(P, (uint32_t)(((uint64_t)x * bal_count(P)) >> 32));
-#pragma unroll
-      for (int p = 0; p < P; ++p) Sm[p][c] = ballot(valid[c] && ((bits >> p) & 1u));
-    }
-#pragma unroll
-    for (int p = 0; p < P; ++p) Bm[p][c] = Sm[p][c];
-    w[c] = valid[c] ? hb[j] : 0;
-  }
-  // w_i = h_i - 2 sum_j G_ij m_j = h_i - sum_j G4_ij (m_j / 2), m_j = sum_p s_pj (even: P is even)
+  for (int c = 0; c < CPL; ++c) w[c] = valid[c] ? hb[c * ARP_WAVE + lane] : 0;
 #pragma unroll
   for (int c2 = 0; c2 < CPL; ++c2) {
     const int nj = D - c2 * ARP_WAVE < ARP_WAVE ? D - c2 * ARP_WAVE : ARP_WAVE;
@@ -73,6 +46,101 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
         for (int c = 0; c < CPL; ++c) w[c] -= half_m * col[c * ARP_WAVE + lane];
       }
     }
+  }
+}
+
+// Start energy up to a constant: 2 E0 + 2 P tr(G) = -(h + w).m (2 E0 = -2 h.m + 2 m^T G m - 2 P tr G and
+// m^T G m = (h - w).m / 2), summed over the wave in 64 bits.
+template <int P, int CPL>
+__device__ __forceinline__ long long start_energy(const int32_t* __restrict__ hb, const bool* valid,
+                                                  uint64_t (&Sm)[P][CPL], const int32_t* w) {
+  const int lane = lane_id();
+  long long e = 0;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    int m = 0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) m += (int)((Sm[p][c] >> lane) & 1ull);
+    m = 2 * m - P;
+    if (valid[c]) e -= (long long)(hb[c * ARP_WAVE + lane] + w[c]) * m;
+  }
+#pragma unroll
+  for (int o = 32; o; o >>= 1) e += __shfl_xor(e, o);
+  return e;
+}
+
+template <int P, int CPL>
+__device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb,
+                                                const int32_t* __restrict__ T, int L, int q, int D, uint32_t r,
+                                                uint32_t inst, uint32_t k0, uint32_t k1, int start, int mode,
+                                                int8_t* __restrict__ out_s, long long* __restrict__ out_de,
+                                                long long* __restrict__ out_st, size_t t,
+                                                const uint32_t* __restrict__ thr, int gate,
+                                                int8_t* __restrict__ used) {
+  const int lane = lane_id();
+  bool valid[CPL];
+  int32_t w[CPL], dj2[CPL];
+  uint64_t vmask[CPL];
+  uint64_t Sm[P][CPL], Bm[P][CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const int j = c * ARP_WAVE + lane;
+    valid[c] = j < D;
+    const int nv = D - c * ARP_WAVE;
+    vmask[c] = nv >= ARP_WAVE ? ~0ull : (nv <= 0 ? 0ull : ((1ull << nv) - 1ull));
+    dj2[c] = valid[c] ? G4[j * D + j] >> 1 : 0;
+    if (thr) {
+      // learned start: spin +1 iff x + thr_pj >= 2^32, x the random start's word; Bm holds the random start for now
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        uint32_t x = 0u, th = 0u;
+        if (valid[c]) {  // lanes past D read nothing: the threshold block holds P*D words per frame
+          x = philox_w0(0xFFFFFFFFu, (uint32_t)(p * D + j), r, inst, k0, k1);
+          th = thr[p * D + j];
+        }
+        Sm[p][c] = ballot(valid[c] && (uint64_t)x + th >= 0x100000000ull);
+        Bm[p][c] = ballot(valid[c] && x >= 0x80000000u);
+      }
+    } else if (start == 0) {
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        Sm[p][c] = ballot(valid[c] &&
+                          philox_w0(0xFFFFFFFFu, (uint32_t)(p * D + j), r, inst, k0, k1) >= 0x80000000u);
+    } else {
+      const uint32_t x = valid[c] ? philox_w0(0xFFFFFFFFu, (uint32_t)j, r, inst, k0, k1) : 0u;
+      const uint32_t bits = bal_pattern(P, (uint32_t)(((uint64_t)x * bal_count(P)) >> 32));
+#pragma unroll
+      for (int p = 0; p < P; ++p) Sm[p][c] = ballot(valid[c] && ((bits >> p) & 1u));
+    }
+    if (!thr) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) Bm[p][c] = Sm[p][c];
+    }
+  }
+  init_fields<P, CPL>(G4, hb, D, valid, Sm, w);
+  if (thr) {
+    // energy gate: keep the learned start unless the random start (in Bm) has lower energy
+    bool take = true;
+    if (gate) {
+      int32_t wr[CPL];
+      init_fields<P, CPL>(G4, hb, D, valid, Bm, wr);
+      const long long el = start_energy<P, CPL>(hb, valid, Sm, w);
+      const long long er = start_energy<P, CPL>(hb, valid, Bm, wr);
+      take = el <= er;
+      if (!take) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          w[c] = wr[c];
+#pragma unroll
+          for (int p = 0; p < P; ++p) Sm[p][c] = Bm[p][c];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+#pragma unroll
+      for (int p = 0; p < P; ++p) Bm[p][c] = Sm[p][c];
+    if (used && lane == 0) used[t] = take ? 1 : 0;
   }
   long long de4 = 0, best4 = 1LL << 60;   // de / 4
   int flips = 0, last = 0;
@@ -191,7 +259,9 @@ __global__ __launch_bounds__(256) void arpeggio_fx_kernel(const int32_t* __restr
                                                           const uint32_t* __restrict__ inst, uint32_t k0, uint32_t k1,
                                                           int start, int mode, int8_t* __restrict__ out_s,
                                                           long long* __restrict__ out_de,
-                                                          long long* __restrict__ out_st) {
+                                                          long long* __restrict__ out_st,
+                                                          const uint32_t* __restrict__ thr, int gate,
+                                                          int8_t* __restrict__ used) {
   extern __shared__ int32_t sG[];
   const int b = blockIdx.x;
   const int nG = D * D;
@@ -202,13 +272,14 @@ __global__ __launch_bounds__(256) void arpeggio_fx_kernel(const int32_t* __restr
   const uint32_t in = inst[b];
   for (int r = wv; r < R; r += W)
     arpeggio_anneal<P, CPL>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)r, in, k0, k1, start, mode, out_s, out_de,
-                            out_st, (size_t)b * R + r);
+                            out_st, (size_t)b * R + r, thr ? thr + (size_t)b * P * D : nullptr, gate, used);
 }
 
 template <int P>
 static hipError_t launch_p(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int R, int D, int B,
                            const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode, int8_t* out_s,
-                           long long* out_de, long long* out_st, int waves, hipStream_t st) {
+                           long long* out_de, long long* out_st, int waves, const uint32_t* thr, int gate,
+                           int8_t* used, hipStream_t st) {
   const int cpl = D <= ARP_WAVE ? 1 : 2;
   const size_t lds = sizeof(int32_t) * ((size_t)D * D + ARP_WAVE * cpl);
   dim3 grid(B), block(ARP_WAVE * waves);
@@ -218,20 +289,21 @@ static hipError_t launch_p(const int32_t* Gt, const int32_t* h, const int32_t* T
   }
   if (cpl == 1)
     hipLaunchKernelGGL((arpeggio_fx_kernel<P, 1>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start,
-                       mode, out_s, out_de, out_st);
+                       mode, out_s, out_de, out_st, thr, gate, used);
   else
     hipLaunchKernelGGL((arpeggio_fx_kernel<P, 2>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start,
-                       mode, out_s, out_de, out_st);
+                       mode, out_s, out_de, out_st, thr, gate, used);
   return hipGetLastError();
 }
 
 hipError_t launch_arpeggio_fx(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R, int D,
                               int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode,
-                              int8_t* out_s, long long* out_de, long long* out_st, int waves, hipStream_t st) {
+                              int8_t* out_s, long long* out_de, long long* out_st, int waves, hipStream_t st,
+                              const uint32_t* thr, int gate, int8_t* used) {
   switch (P) {
-    case 2: return launch_p<2>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, st);
-    case 4: return launch_p<4>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, st);
-    default: return launch_p<6>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, st);
+    case 2: return launch_p<2>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, thr, gate, used, st);
+    case 4: return launch_p<4>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, thr, gate, used, st);
+    default: return launch_p<6>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, thr, gate, used, st);
   }
 }
 

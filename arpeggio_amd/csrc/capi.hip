@@ -99,7 +99,7 @@ extern "C" int arp_anneal_chord(const int16_t* J, const int32_t* h, const int32_
 extern "C" int arp_front_end(const float* H, const float* y, const float* N0, float N0_scalar, int B, int Mr, int D,
                              int M, int K, float Q, int8_t* xg, int32_t* Gi, int32_t* hi, float* scale, float* Gf,
                              float* hf, int16_t* J, int32_t* hN, float* z, float* mu, float* Jf, float* hNf,
-                             void* stream) {
+                             uint16_t* feat, int feat_ld, void* stream) {
   if (D < 1 || D > 128) return fail(ARP_E_RANGE, "front_end: D = %d outside 1..128", D);
   if (Mr < 1 || Mr > 128) return fail(ARP_E_RANGE, "front_end: Mr = %d outside 1..128", Mr);
   if (M != 4 && M != 16) return fail(ARP_E_ARG, "front_end: M = %d, expected 4 or 16", M);
@@ -108,7 +108,9 @@ extern "C" int arp_front_end(const float* H, const float* y, const float* N0, fl
   if (B < 0) return fail(ARP_E_ARG, "front_end: B = %d < 0", B);
   if (B == 0) return ok();
   if (!H || !y) return fail(ARP_E_ARG, "front_end: null input");
-  arp::FrontOut o{xg, Gi, hi, scale, Gf, hf, J, hN, z, mu, Jf, hNf};
+  if (feat && feat_ld < arp::feat_count(D))
+    return fail(ARP_E_ARG, "front_end: feat_ld = %d < %d features", feat_ld, arp::feat_count(D));
+  arp::FrontOut o{xg, Gi, hi, scale, Gf, hf, J, hN, z, mu, Jf, hNf, feat, feat_ld};
   return hip_check(arp::launch_front_end(H, y, N0, N0_scalar, B, Mr, D, M, K, Q, o, (hipStream_t)stream),
                    "front_end launch");
 }
@@ -156,4 +158,72 @@ extern "C" int arp_anneal_chord_f32(const float* J, const float* h, const float*
   return hip_check(arp::launch_chord_f32(J, h, h0, T, c, q, S, P, R, N, B, k0, k1, inst0, out_e, out_s, out_u,
                                          (long long*)out_fl, waves, device_max_lds(), (hipStream_t)stream),
                    "chord_f32 launch");
+}
+
+extern "C" int arp_anneal_arpeggio_ws(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R,
+                                      int D, int B, const uint32_t* inst, uint32_t k0, uint32_t k1,
+                                      const uint32_t* thr, int gate, int8_t* out_s, int64_t* out_de, int64_t* out_st,
+                                      int8_t* out_used, void* stream) {
+  if (P != 2 && P != 4 && P != 6) return fail(ARP_E_ARG, "arpeggio_ws: P = %d, expected 2, 4 or 6", P);
+  if (D < 1 || D > 128) return fail(ARP_E_RANGE, "arpeggio_ws: D = %d outside 1..128", D);
+  if (L < 1 || R < 1 || B < 0) return fail(ARP_E_ARG, "arpeggio_ws: L = %d, R = %d, B = %d", L, R, B);
+  if (B == 0) return ok();
+  if (!Gt || !h || !T || !inst || !thr || !out_s || !out_de || !out_st)
+    return fail(ARP_E_ARG, "arpeggio_ws: null pointer");
+  const int waves = R < 4 ? R : 4;
+  return hip_check(arp::launch_arpeggio_fx(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, 0, 0, out_s, (long long*)out_de,
+                                           (long long*)out_st, waves, (hipStream_t)stream, thr, gate ? 1 : 0,
+                                           out_used),
+                   "arpeggio_ws launch");
+}
+
+extern "C" int arp_mlp_gemm(const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, int M, int N,
+                            int K, const float* bias, int act, float* Cf, float* Cpre, uint16_t* Cb, long ldc,
+                            void* stream) {
+  if (M < 0 || N < 0 || K < 1) return fail(ARP_E_ARG, "mlp_gemm: M = %d, N = %d, K = %d", M, N, K);
+  if (act != 0 && act != 1) return fail(ARP_E_ARG, "mlp_gemm: act = %d", act);
+  if (M == 0 || N == 0) return ok();
+  if (!A || !B || (!Cf && !Cpre && !Cb)) return fail(ARP_E_ARG, "mlp_gemm: null pointer");
+  return hip_check(arp::launch_gemm_bf16(A, sam, sak, B, sbk, sbn, M, N, K, bias, act, Cf, Cpre, Cb, ldc,
+                                         (hipStream_t)stream),
+                   "mlp_gemm launch");
+}
+
+extern "C" int arp_mlp_gelu_bwd(const float* dH, const float* Apre, long n, uint16_t* dAb, float* dAf, void* stream) {
+  if (n < 0 || !dH || !Apre || (!dAb && !dAf)) return fail(ARP_E_ARG, "mlp_gelu_bwd: bad arguments");
+  if (n == 0) return ok();
+  return hip_check(arp::launch_gelu_bwd(dH, Apre, n, dAb, dAf, (hipStream_t)stream), "gelu_bwd launch");
+}
+
+extern "C" int arp_mlp_bce(const float* z, long ld, const int8_t* y, int M, int N, uint16_t* dzb, float* dzf,
+                           float* loss, void* stream) {
+  if (M < 0 || N < 0 || !z || !y) return fail(ARP_E_ARG, "mlp_bce: bad arguments");
+  if (M == 0 || N == 0) return ok();
+  return hip_check(arp::launch_bce(z, ld, y, M, N, dzb, dzf, loss, (hipStream_t)stream), "bce launch");
+}
+
+extern "C" int arp_mlp_colsum(const float* X, long ld, int M, int N, float* out, void* stream) {
+  if (M < 0 || N < 0 || !X || !out) return fail(ARP_E_ARG, "mlp_colsum: bad arguments");
+  if (N == 0) return ok();
+  return hip_check(arp::launch_colsum(X, ld, M, N, out, (hipStream_t)stream), "colsum launch");
+}
+
+extern "C" int arp_mlp_adam(float* W, const float* G, float* m, float* v, long n, float lr, float b1, float b2,
+                            float eps, int step, uint16_t* Wb, void* stream) {
+  if (n < 0 || !W || !G || !m || !v || step < 1) return fail(ARP_E_ARG, "mlp_adam: bad arguments");
+  if (n == 0) return ok();
+  return hip_check(arp::launch_adam(W, G, m, v, n, lr, b1, b2, eps, step, Wb, (hipStream_t)stream), "adam launch");
+}
+
+extern "C" int arp_cast_bf16(const float* x, long n, uint16_t* y, void* stream) {
+  if (n < 0 || !x || !y) return fail(ARP_E_ARG, "cast_bf16: bad arguments");
+  if (n == 0) return ok();
+  return hip_check(arp::launch_cast_bf16(x, n, y, (hipStream_t)stream), "cast launch");
+}
+
+extern "C" int arp_start_thresholds(const float* z, long ld, int M, int N, float inv_temp, uint32_t* thr,
+                                    void* stream) {
+  if (M < 0 || N < 0 || !z || !thr) return fail(ARP_E_ARG, "start_thresholds: bad arguments");
+  if (M == 0 || N == 0) return ok();
+  return hip_check(arp::launch_start_thr(z, ld, M, N, inv_temp, thr, (hipStream_t)stream), "start_thr launch");
 }

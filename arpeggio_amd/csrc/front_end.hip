@@ -21,6 +21,11 @@ namespace arp {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
 __device__ __forceinline__ float block_max(float v, float* red) {
   for (int o = 32; o; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
   const int wv = threadIdx.x >> 6;
@@ -74,6 +79,7 @@ __global__ __launch_bounds__(256) void front_end_kernel(const float* __restrict_
   float* xv = uv + Dp;   // x_g (float levels)
   float* hv = xv + Dp;   // h
   float* tv = hv + Dp;   // saved column of L
+  float* zv_keep = tv + Dp;  // z / mu (learned-start features)
   float* red = tv + 2 * Dp;
   const int tid = threadIdx.x, nt = blockDim.x;
   const float* Hb = H + (size_t)b * Mr * D;
@@ -143,6 +149,7 @@ __global__ __launch_bounds__(256) void front_end_kernel(const float* __restrict_
     l = fminf(fmaxf(l, -lmax), lmax);
     if (v != v) l = -lmax;
     xv[i] = l;
+    zv_keep[i] = v;
     if (o.z) o.z[(size_t)b * D + i] = s;
     if (o.mu) o.mu[(size_t)b * D + i] = mu;
     if (o.xg) o.xg[(size_t)b * D + i] = (int8_t)l;
@@ -192,6 +199,26 @@ __global__ __launch_bounds__(256) void front_end_kernel(const float* __restrict_
       const int r = x / N, c = x - r * N;
       o.Jf[(size_t)b * N * N + x] = r == c ? 0.f : -(As[(r % D) * Ap + (c % D)] / scale);
     }
+  }
+  if (o.feat) {
+    float gd = 0.f;
+    for (int j = tid; j < D; j += nt) gd += As[j * Ap + j];
+    for (int off = 32; off; off >>= 1) gd += __shfl_xor(gd, off);
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = gd;
+    __syncthreads();
+    float gs = 0.f;
+    for (int i = 0; i < (nt >> 6); ++i) gs += red[i];
+    const float gbar = gs / (float)D;
+    uint16_t* f = o.feat + (size_t)b * o.feat_ld;
+    for (int j = tid; j < D; j += nt) {
+      const float v = zv_keep[j];
+      f[j] = to_bf16(cv[j] / gbar);
+      f[D + j] = to_bf16(As[j * Ap + j] / gbar - 1.f);
+      f[2 * D + j] = to_bf16(xv[j] / lmax);
+      f[3 * D + j] = to_bf16(v - xv[j]);
+    }
+    for (int x = 4 * D + tid; x < o.feat_ld; x += nt) f[x] = to_bf16(x == 4 * D ? sigma / gbar : 0.f);
   }
   if (o.J) {
     // J[(p,i),(p',j)] = -G_int[i][j] off the diagonal, 0 on it (chord_ref.expand)
@@ -327,6 +354,19 @@ __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restri
     if (o.z) o.z[(size_t)b * D + lane] = z;
     if (o.mu) o.mu[(size_t)b * D + lane] = mu;
     if (o.xg) o.xg[(size_t)b * D + lane] = (int8_t)l;
+  }
+  if (o.feat) {
+    float gd = valid ? Gs[lane * ld + lane] : 0.f;
+    for (int off = 32; off; off >>= 1) gd += __shfl_xor(gd, off);
+    const float gbar = gd / (float)D;
+    uint16_t* f = o.feat + (size_t)b * o.feat_ld;
+    if (valid) {
+      f[lane] = to_bf16(c / gbar);
+      f[D + lane] = to_bf16(Gs[lane * ld + lane] / gbar - 1.f);
+      f[2 * D + lane] = to_bf16(l / lmax);
+      f[3 * D + lane] = to_bf16(v - l);
+    }
+    for (int x = 4 * D + lane; x < o.feat_ld; x += 64) f[x] = to_bf16(x == 4 * D ? sigma / gbar : 0.f);
   }
   // h_j = 2 (c_j - (G x_g)_j); scale = max(max|h|, max|G|)
   colb[lane] = l;

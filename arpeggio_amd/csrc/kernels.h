@@ -17,11 +17,18 @@ struct FrontOut {
   float* mu;       // (B, D)
   float* Jf;       // (B, N, N) expanded dDI couplings, float (G / scale)
   float* hNf;      // (B, N)
+  uint16_t* feat;  // (B, feat_ld) bf16 learned-start features: H^T y / g, G_jj / g - 1, x_g / lmax, z/mu - x_g,
+                   // (N0/2) / g, zero padding; g = mean_j G_jj
+  int feat_ld;
 };
+
+// number of learned-start features for D coordinates (before padding)
+__host__ __device__ inline int feat_count(int D) { return 4 * D + 1; }
 
 hipError_t launch_arpeggio_fx(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R, int D,
                               int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode,
-                              int8_t* out_s, long long* out_de, long long* out_st, int waves, hipStream_t st);
+                              int8_t* out_s, long long* out_de, long long* out_st, int waves, hipStream_t st,
+                              const uint32_t* thr = nullptr, int gate = 0, int8_t* used = nullptr);
 hipError_t launch_chord_fx(const int16_t* J, const int32_t* h, const int32_t* h0, const int32_t* Tq,
                            const int32_t* Cq, int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1,
                            int inst0, int mode, int32_t* out_e, int8_t* out_s, unsigned long long* stats, int waves,
@@ -39,5 +46,17 @@ hipError_t launch_chord_f32(const float* J, const float* h, const float* h0, con
                             int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1, int inst0, float* out_e,
                             int8_t* out_s, float* out_u, long long* out_fl, int waves, size_t max_lds,
                             hipStream_t st);
+
+hipError_t launch_gemm_bf16(const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, int M, int N,
+                            int K, const float* bias, int act, float* Cf, float* Cpre, uint16_t* Cb, long ldc,
+                            hipStream_t st);
+hipError_t launch_gelu_bwd(const float* dH, const float* Apre, long n, uint16_t* dAb, float* dAf, hipStream_t st);
+hipError_t launch_bce(const float* z, long ld, const int8_t* y, int M, int N, uint16_t* dzb, float* dzf,
+                      float* loss_acc, hipStream_t st);
+hipError_t launch_colsum(const float* X, long ld, int M, int N, float* out, hipStream_t st);
+hipError_t launch_adam(float* W, const float* G, float* m, float* v, long n, float lr, float b1, float b2, float eps,
+                       int step, uint16_t* Wb, hipStream_t st);
+hipError_t launch_cast_bf16(const float* x, long n, uint16_t* y, hipStream_t st);
+hipError_t launch_start_thr(const float* z, long ld, int M, int N, float inv_temp, uint32_t* thr, hipStream_t st);
 
 }  // namespace arp

@@ -46,7 +46,7 @@ def front_end(H, y, N0, M, K=1, Q=QSCALE, want=('xg', 'Gi', 'hi', 'scale')):
     order = ('xg', 'Gi', 'hi', 'scale', 'Gf', 'hf', 'J', 'hN', 'z', 'mu', 'Jf', 'hNf')
     L = _lib.load()
     _lib.check(L.arp_front_end(_lib.ptr(H), _lib.ptr(y), _lib.ptr(n0t), n0s, B, Mr, D, int(M), int(K), float(Q),
-                               *[_lib.ptr(out.get(k)) for k in order], _lib.stream_ptr()), 'front_end')
+                               *[_lib.ptr(out.get(k)) for k in order], None, 0, _lib.stream_ptr()), 'front_end')
     return out
 
 
@@ -79,7 +79,10 @@ class DetectPipeline:
     policy 'chord': snapshot anneal on the expanded N = 2D problem, cycles = S steps, schedule (t0, t1, q, kappa)."""
 
     def __init__(self, B, Mr, D, M, R, policy='arpeggio', cycles=64, T0=0.1, T1=0.1, q=0.0, kappa=0.0,
-                 start='rand', key=0, inst0=0, mode='arpeggio', Q=QSCALE):
+                 start='rand', key=0, inst0=0, mode='arpeggio', Q=QSCALE, model=None, gate=True, inv_temp=1.0):
+        """start 'learned' (Arpeggio only) needs model (a learned.WarmStartMLP): the front end also writes the
+        features, the MLP turns them into per-spin start thresholds, and the anneal starts from them (gate: fall back
+        to the random start per anneal when its energy is lower)."""
         self.B, self.Mr, self.D, self.M, self.R = B, Mr, D, M, R
         self.policy, self.K, self.P = policy, 1, 2
         self.key, self.inst0, self.start, self.mode = key, inst0, start, mode
@@ -101,6 +104,13 @@ class DetectPipeline:
             self.spins = torch.empty(B, R, N, dtype=torch.int8, device=dev)
             self.de = torch.empty(B, R, dtype=torch.int64, device=dev)
             self.st = torch.empty(B, R, 2, dtype=torch.int64, device=dev)
+            if start == 'learned':
+                from .learned import feat_ld
+                assert model is not None and model.D == D and model.P == self.P, 'learned start needs a matching model'
+                self.model, self.gate, self.inv_temp = model, gate, inv_temp
+                self.feat = torch.empty(B, feat_ld(D), dtype=torch.bfloat16, device=dev)
+                self.thr = torch.empty(B, N, dtype=torch.int32, device=dev)
+                self.used = torch.empty(B, R, dtype=torch.int8, device=dev)
         elif policy == 'chord':
             assert N <= 256
             dummy = dict(J=torch.zeros(1, 2, 2), h=torch.zeros(1, 2), h0=torch.zeros(1, 2))
@@ -120,18 +130,27 @@ class DetectPipeline:
         B, Mr, D, M, R, P = self.B, self.Mr, self.D, self.M, self.R, self.P
         n0t, n0s = _n0_arg(N0, B)
         P_ = _lib.ptr
-        if self.policy == 'arpeggio':
+        if self.policy == 'arpeggio' and self.start == 'learned':
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
-                                       P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None, None, None, s),
-                       'front_end')
+                                       P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None,
+                                       None, None, P_(self.feat), self.feat.shape[1], s), 'front_end')
+            self.model.thresholds(self.feat, self.inv_temp, out=self.thr)
+            _lib.check(L.arp_anneal_arpeggio_ws(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D,
+                                                B, P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
+                                                P_(self.thr), 1 if self.gate else 0, P_(self.spins), P_(self.de),
+                                                P_(self.st), P_(self.used), s), 'arpeggio_ws')
+        elif self.policy == 'arpeggio':
+            _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
+                                       P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None,
+                                       None, None, None, 0, s), 'front_end')
             _lib.check(L.arp_anneal_arpeggio(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D, B,
                                              P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
                                              0 if self.start == 'rand' else 1, 0 if self.mode == 'arpeggio' else 1,
                                              P_(self.spins), P_(self.de), P_(self.st), s), 'arpeggio')
         else:
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg), None,
-                                       None, P_(self.scale), None, None, P_(self.J), P_(self.hN), None, None, None, None, s),
-                       'front_end')
+                                       None, P_(self.scale), None, None, P_(self.J), P_(self.hN), None, None, None,
+                                       None, None, 0, s), 'front_end')
             _lib.check(L.arp_anneal_chord(P_(self.J), P_(self.hN), P_(self.hN), P_(self.T), P_(self.c), self.q, self.S,
                                           P, R, P * D, B, self.key & MASK, (self.key >> 32) & MASK, self.inst0, 0,
                                           P_(self.e), P_(self.spins), None, s), 'chord')

@@ -1,0 +1,87 @@
+#!/usr/bin/env python3
+"""Trains the learned warm start on the GPU and evaluates it against the plain random start.
+
+    python -m arpeggio_amd.train_warmstart [--frames 1048576] [--out arpeggio_amd/weights/....npz]
+
+Training frames: 32 users x 32 rx, 16-QAM, i.i.d. Rayleigh, per-frame SNR uniform in [snr-lo, snr-hi] dB, drawn on the
+GPU. Labels: the detection of a 1024-pass, 8-anneal first-accept run (learned.label_batch). Evaluation: held-out
+frames from the reference's host draw recipe (seed < 1000) at one SNR; bit error rates of the plain start and the
+learned start at several pass budgets, R anneals per frame. Prints one JSON line."""
+import argparse
+import json
+import os
+import sys
+import time
+
+import torch
+
+from . import frames
+from .detector import DetectPipeline
+from .learned import DEFAULT_WEIGHTS, WarmStartMLP, label_batch, train
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--frames', type=int, default=1 << 20)
+    ap.add_argument('--chunk', type=int, default=65536)
+    ap.add_argument('--snr-lo', type=float, default=20.0)
+    ap.add_argument('--snr-hi', type=float, default=28.0)
+    ap.add_argument('--passes', type=int, default=1024)
+    ap.add_argument('--label-R', type=int, default=8)
+    ap.add_argument('--epochs', type=int, default=4)
+    ap.add_argument('--batch', type=int, default=8192)
+    ap.add_argument('--lr', type=float, default=2e-3)
+    ap.add_argument('--seed', type=int, default=2024)
+    ap.add_argument('--out', default=DEFAULT_WEIGHTS)
+    ap.add_argument('--eval-snr', type=float, default=24.0)
+    ap.add_argument('--eval-frames', type=int, default=65536)
+    ap.add_argument('--R', type=int, default=16)
+    a = ap.parse_args()
+    log = lambda *x: print('#', *x, file=sys.stderr, flush=True)
+    t0 = time.time()
+    feats, labels = [], []
+    for c in range(0, a.frames, a.chunk):
+        B = min(a.chunk, a.frames - c)
+        b = frames.draw_device(a.seed * 7919 + c, a.snr_lo, a.snr_hi, B, 32, 16)
+        f, l, be = label_batch(b['H'], b['y'], b['N0'], b['l'].to(torch.int8), R=a.label_R, passes=a.passes,
+                               key=a.seed * 0x9E3779B9 + c, inst0=c)
+        feats.append(f)
+        labels.append(l)
+        if c == 0:
+            log('labels: BER of the labelling run on chunk 0: %.3e' % (float(be.sum()) / (B * 128)))
+    feats = torch.cat(feats)
+    labels = torch.cat(labels)
+    torch.cuda.synchronize()
+    t_label = time.time() - t0
+    log('labelled %d frames in %.1f s' % (feats.shape[0], t_label))
+    model = WarmStartMLP(64, seed=a.seed)
+    t1 = time.time()
+    losses = train(model, feats, labels, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+    torch.cuda.synchronize()
+    t_train = time.time() - t1
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    model.save(a.out)
+    log('trained in %.1f s, saved %s' % (t_train, a.out))
+    # held-out evaluation on the reference's draw recipe
+    b = frames.draw(7, a.eval_snr, 0, a.eval_frames, 32, 16, dev='cuda')
+    lt = b['l'].to(torch.int8)
+    nbits = a.eval_frames * 128
+    res = {}
+    for L in (8, 16, 32, 64):
+        for start in ('rand', 'learned'):
+            pipe = DetectPipeline(a.eval_frames, 64, 64, 16, a.R, policy='arpeggio', cycles=L, T0=0.1, T1=0.1, q=0.0,
+                                  start=start, key=99, model=model if start == 'learned' else None)
+            _, be = pipe.run(b['H'], b['y'], b['N0'], lt)
+            torch.cuda.synchronize()
+            r = dict(ber=float(be.sum()) / nbits, flips=float(pipe.st[..., 0].double().mean()))
+            if start == 'learned':
+                r['learned_kept'] = float(pipe.used.double().mean())
+            res['%s_L%d' % (start, L)] = r
+            log(start, L, r)
+    print(json.dumps(dict(frames=a.frames, label_passes=a.passes, label_R=a.label_R, epochs=a.epochs,
+                          loss_first=losses[0], loss_last=losses[-1], label_s=t_label, train_s=t_train,
+                          eval_snr=a.eval_snr, eval_frames=a.eval_frames, R=a.R, results=res)), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -113,7 +113,7 @@ def main():
         s = _lib.stream_ptr(stream)
         _lib.check(L_.arp_front_end(P_(H), P_(y), None, float(b['N0']), B, 2 * nt, 2 * nt, M, 1, pipe.Q, P_(pipe.xg),
                                     P_(pipe.Gi), P_(pipe.hi), P_(pipe.scale), None, None, None, None, None, None, None, None,
-                                    s),
+                                    None, 0, s),
                    'front_end')
         if i is not None:
             ev[i][0].record(stream)

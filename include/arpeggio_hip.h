@@ -70,10 +70,13 @@ int arp_anneal_chord(const int16_t *J, const int32_t *h, const int32_t *h0, cons
  * Outputs (each may be NULL): xg (B, D) int8 levels; Gi (B, D, D) int32 = rne(G / scale Q) (symmetric, so also the
  * transposed Gram the Arpeggio kernel takes); hi (B, D) int32; scale (B); Gf (B, D, D), hf (B, D) float32 G/scale,
  * h/scale; J (B, N, N) int16 and hN (B, N) int32, the expanded Chord inputs (N = 2 K D <= 256); z, mu (B, D);
- * Jf (B, N, N), hNf (B, N) float32, the expanded inputs of the FP32 Chord anneal. */
+ * Jf (B, N, N), hNf (B, N) float32, the expanded inputs of the FP32 Chord anneal; feat (B, feat_ld) bf16 (bit
+ * pattern as uint16), the learned-start features [H^T y / g, G_jj / g - 1, x_g / lmax, z/mu - x_g (D each),
+ * (N0/2) / g, zeros], g = mean_j G_jj, feat_ld >= 4 D + 1 (no reference counterpart). */
 int arp_front_end(const float *H, const float *y, const float *N0, float N0_scalar, int B, int Mr, int D, int M, int K,
                   float Q, int8_t *xg, int32_t *Gi, int32_t *hi, float *scale, float *Gf, float *hf, int16_t *J,
-                  int32_t *hN, float *z, float *mu, float *Jf, float *hNf, void *stream);
+                  int32_t *hN, float *z, float *mu, float *Jf, float *hNf, uint16_t *feat, int feat_ld,
+                  void *stream);
 
 /* Decode and select: candidate levels from anneal spins (delta = 1: clip(x_g + sum of the P copies); delta = 0: RI
  * spins), lowest ||y - H x||^2 over the R candidates then x_g (first minimum), Gray bit errors against ltrue.
@@ -99,6 +102,39 @@ int arp_anneal_arpeggio_f32(const float *Gt, const float *h, const float *T, int
 int arp_anneal_chord_f32(const float *J, const float *h, const float *h0, const float *T, const float *c, float q, int S,
                          int P, int R, int N, int B, uint32_t k0, uint32_t k1, int inst0, float *out_e, int8_t *out_s,
                          float *out_u, int64_t *out_fl, void *stream);
+
+/* Fixed-point first-accept anneal from a learned start (no reference counterpart; the reference's start is the random
+ * one of arpeggio_fx.cu:48-51, kept here as the fallback). Arguments as arp_anneal_arpeggio, plus
+ *   thr (B, P*D) uint32 per-spin start thresholds: spin +1 iff x + thr >= 2^32, x the random start's Philox word
+ *       (thr = 2^31 reproduces the random start exactly)
+ *   gate 1: per anneal, keep the learned start only if its Ising energy is not above the random start's
+ *   out_used (B, R) int8 or NULL: 1 where the learned start was kept. */
+int arp_anneal_arpeggio_ws(const int32_t *Gt, const int32_t *h, const int32_t *T, int L, int q, int P, int R, int D,
+                           int B, const uint32_t *inst, uint32_t k0, uint32_t k1, const uint32_t *thr, int gate,
+                           int8_t *out_s, int64_t *out_de, int64_t *out_st, int8_t *out_used, void *stream);
+
+/* Learned-start MLP kernels (no reference counterpart). bf16 tensors are passed as their uint16 bit patterns.
+ * arp_mlp_gemm: C(m, n) = sum_k A(m, k) B(k, n) (+ bias[n]), act 0 none / 1 GELU (tanh form), on v_mfma_f32_16x16x32_bf16
+ *   with fp32 accumulation; A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]; outputs (each may be NULL): Cf
+ *   fp32 after the activation, Cpre fp32 before it, Cb bf16 after it, row stride ldc. */
+int arp_mlp_gemm(const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn, int M, int N, int K,
+                 const float *bias, int act, float *Cf, float *Cpre, uint16_t *Cb, long ldc, void *stream);
+/* dA = dH * GELU'(Apre) over n elements, to bf16 and/or fp32. */
+int arp_mlp_gelu_bwd(const float *dH, const float *Apre, long n, uint16_t *dAb, float *dAf, void *stream);
+/* Mean binary cross-entropy of logits z (M x N, row stride ld) against spins y (int8, +-1, M x N contiguous):
+ * gradient (sigmoid(z) - (y+1)/2) / (M N) to bf16 and/or fp32; *loss += the mean loss (atomic). */
+int arp_mlp_bce(const float *z, long ld, const int8_t *y, int M, int N, uint16_t *dzb, float *dzf, float *loss,
+                void *stream);
+/* out[n] = sum over m of X[m*ld + n] (bias gradients). */
+int arp_mlp_colsum(const float *X, long ld, int M, int N, float *out, void *stream);
+/* One Adam step (bias-corrected, step >= 1) on fp32 master weights W with gradient G and moments m, v; Wb (or NULL)
+ * receives the bf16 copy. */
+int arp_mlp_adam(float *W, const float *G, float *m, float *v, long n, float lr, float b1, float b2, float eps, int step,
+                 uint16_t *Wb, void *stream);
+/* fp32 -> bf16, round to nearest even. */
+int arp_cast_bf16(const float *x, long n, uint16_t *y, void *stream);
+/* Start thresholds from logits: thr = rint(sigmoid(z * inv_temp) 2^32) clamped to [1, 2^32 - 1]. */
+int arp_start_thresholds(const float *z, long ld, int M, int N, float inv_temp, uint32_t *thr, void *stream);
 
 #ifdef __cplusplus
 }

@@ -134,15 +134,67 @@ typedef struct {
     int64_t best, flips, tests, n1, init, n_acc;
 } arp_result;
 
+/* Fields and start energy of a copy-major state: w_i = h_i - 2 sum_j G_ij m_j; returns -(h + w).m, which is
+ * 2 E0 + 2 P tr(G) of the dDI expansion (2 E0 = -2 h.m + 2 m^T G m - 2 P tr G, and m^T G m = (h - w).m / 2). */
+static int64_t fields_of(const int32_t *G, const int32_t *h, int D, int P, const int8_t *s, int64_t *m, int64_t *w)
+{
+    for (int j = 0; j < D; j++) {
+        int64_t v = 0;
+        for (int p = 0; p < P; p++) v += s[p * D + j];
+        m[j] = v;
+    }
+    int64_t e = 0;
+    for (int i = 0; i < D; i++) {
+        int64_t acc = 0;
+        for (int j = 0; j < D; j++) acc += (int64_t)G[(size_t)i * D + j] * m[j];
+        w[i] = (int64_t)h[i] - 2 * acc;
+        e -= ((int64_t)h[i] + w[i]) * m[i];
+    }
+    return e;
+}
+
+/* thr (P D, optional): learned start, spin +1 iff Philox(2^32 - 1, p D + j, r, inst)[0] >= 2^32 - thr_pj (thr = 2^31
+ * is the random start). gate: take the learned start only if its energy is not above the random start's.
+ * used (optional): 1 if the learned start was taken. */
+static void arpeggio_one_ws(const int32_t *G, const int32_t *h, int D, int P, const int32_t *T, int L, int q,
+                            uint32_t r, uint32_t inst, uint32_t k0, uint32_t k1, int start, int mode, int8_t *bs,
+                            arp_result *res, int32_t *acc_k, int32_t *acc_j, long max_acc, int32_t *tests_k,
+                            const uint32_t *thr, int gate, int8_t *used);
+
 static void arpeggio_one(const int32_t *G, const int32_t *h, int D, int P, const int32_t *T, int L, int q, uint32_t r,
                          uint32_t inst, uint32_t k0, uint32_t k1, int start, int mode, int8_t *bs, arp_result *res,
                          int32_t *acc_k, int32_t *acc_j, long max_acc, int32_t *tests_k)
+{
+    arpeggio_one_ws(G, h, D, P, T, L, q, r, inst, k0, k1, start, mode, bs, res, acc_k, acc_j, max_acc, tests_k, NULL, 0,
+                    NULL);
+}
+
+static void arpeggio_one_ws(const int32_t *G, const int32_t *h, int D, int P, const int32_t *T, int L, int q,
+                            uint32_t r, uint32_t inst, uint32_t k0, uint32_t k1, int start, int mode, int8_t *bs,
+                            arp_result *res, int32_t *acc_k, int32_t *acc_j, long max_acc, int32_t *tests_k,
+                            const uint32_t *thr, int gate, int8_t *used)
 {
     int8_t *s = (int8_t *)malloc((size_t)P * D);
     int64_t *w = (int64_t *)malloc(sizeof(int64_t) * D);
     int64_t *m = (int64_t *)malloc(sizeof(int64_t) * D);
     uint8_t *fl = (uint8_t *)malloc((size_t)D);
-    if (start == 0) {
+    if (thr) {
+        int8_t *sr = (int8_t *)malloc((size_t)P * D);
+        for (int p = 0; p < P; p++)
+            for (int j = 0; j < D; j++) {
+                uint32_t x = philox_word(MASK32, (uint32_t)(p * D + j), r, inst, k0, k1, 0);
+                s[p * D + j] = (uint64_t)x + thr[p * D + j] >= 0x100000000ull ? 1 : -1;
+                sr[p * D + j] = x >= 0x80000000u ? 1 : -1;
+            }
+        int take = 1;
+        if (gate) {
+            int64_t el = fields_of(G, h, D, P, s, m, w), er = fields_of(G, h, D, P, sr, m, w);
+            take = el <= er;
+        }
+        if (!take) memcpy(s, sr, (size_t)P * D);
+        if (used) *used = (int8_t)take;
+        free(sr);
+    } else if (start == 0) {
         for (int p = 0; p < P; p++)
             for (int j = 0; j < D; j++)
                 s[p * D + j] = philox_word(MASK32, (uint32_t)(p * D + j), r, inst, k0, k1, 0) >= 0x80000000u ? 1 : -1;
@@ -275,6 +327,42 @@ static void arp_batch_item(void *vctx, long t)
     c->out_de[t] = res.best;
     c->out_st[2 * t] = res.flips;
     c->out_st[2 * t + 1] = res.tests;
+}
+
+typedef struct {
+    const int32_t *G, *h, *T;
+    const uint32_t *inst, *thr;
+    int D, P, R, L, q, gate;
+    uint32_t k0, k1;
+    int8_t *out_s, *used;
+    int64_t *out_de, *out_st;
+} arp_ws_ctx;
+
+static void arp_ws_item(void *vctx, long t)
+{
+    arp_ws_ctx *c = (arp_ws_ctx *)vctx;
+    long b = t / c->R;
+    uint32_t r = (uint32_t)(t % c->R);
+    arp_result res;
+    arpeggio_one_ws(c->G + (size_t)b * c->D * c->D, c->h + (size_t)b * c->D, c->D, c->P, c->T, c->L, c->q, r,
+                    c->inst[b], c->k0, c->k1, 0, 0, c->out_s + (size_t)t * c->P * c->D, &res, NULL, NULL, 0, NULL,
+                    c->thr + (size_t)b * c->P * c->D, c->gate, c->used ? c->used + t : NULL);
+    c->out_de[t] = res.best;
+    c->out_st[2 * t] = res.flips;
+    c->out_st[2 * t + 1] = res.tests;
+}
+
+/* First-accept anneal from a learned start: thr (B, P D) uint32 per-spin thresholds; gate 1 = energy-gated against
+ * the random start. used (B, R) int8 (optional): 1 where the learned start was taken. */
+int orc_arpeggio_ws(const int32_t *G, const int32_t *h, int B, int D, int P, int R, const int32_t *T, int L, int q,
+                    const uint32_t *inst, uint64_t key, const uint32_t *thr, int gate, int8_t *out_s, int64_t *out_de,
+                    int64_t *out_st, int8_t *used, int nthreads)
+{
+    if (D < 1 || L < 1 || R < 1 || B < 0 || (P != 2 && P != 4 && P != 6) || !thr) return -1;
+    arp_ws_ctx c = {G, h, T, inst, thr, D, P, R, L, q, gate, (uint32_t)(key & MASK32), (uint32_t)(key >> 32), out_s,
+                    used, out_de, out_st};
+    run_pool(arp_ws_item, &c, (long)B * R, nthreads);
+    return 0;
 }
 
 /* arpeggiofx.solve semantics (arpeggiofx.py:23-44): G (B, D, D) row-major, h (B, D), inst (B) uint32, key 64-bit

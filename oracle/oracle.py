@@ -54,6 +54,7 @@ def lib():
         L.orc_arpeggio_fx.argtypes = [P, P, i32, i32, i32, i32, P, i32, i32, P, u64, i32, i32, P, P, P, i32]
         L.orc_arpeggio_trace.argtypes = [P, P, i32, i32, P, i32, i32, u32, u32, u64, i32, i32, P, P, P, P,
                                          ctypes.c_long, P]
+        L.orc_arpeggio_ws.argtypes = [P, P, i32, i32, i32, i32, P, i32, i32, P, u64, P, i32, P, P, P, P, i32]
         L.orc_chord_fx.argtypes = [P, P, P, i32, i32, P, P, i32, i32, i32, i32, u64, i64, i32, P, P, P, i32]
         L.orc_arpeggio_float_f32.argtypes = [P, P, i32, i32, i32, i32, P, i32, ctypes.c_float, P, u64, i32, P, P, P,
                                              P, i32]
@@ -102,6 +103,25 @@ def arpeggio_fx(G, h, P, R, T, q, inst, key, start, mode='arpeggio', nthreads=No
                                _threads(nthreads))
     assert rc == 0
     return s, de, st
+
+
+def arpeggio_ws(G, h, P, R, T, q, inst, key, thr, gate=True, nthreads=None):
+    """First-accept anneal from per-spin start thresholds thr (B, P D) uint32, energy-gated against the random start.
+    Returns spins, de, st and used (B, R) int8."""
+    G = np.ascontiguousarray(G, dtype=np.int32)
+    h = np.ascontiguousarray(h, dtype=np.int32)
+    B, D = h.shape
+    T = np.ascontiguousarray(T, dtype=np.int32)
+    inst = np.ascontiguousarray(np.asarray(inst, dtype=np.int64) & MASK32, dtype=np.uint32)
+    thr = np.ascontiguousarray(thr, dtype=np.uint32)
+    s = np.zeros((B, R, P * D), dtype=np.int8)
+    de = np.zeros((B, R), dtype=np.int64)
+    st = np.zeros((B, R, 2), dtype=np.int64)
+    used = np.zeros((B, R), dtype=np.int8)
+    rc = lib().orc_arpeggio_ws(_p(G), _p(h), B, D, P, R, _p(T), len(T), int(q), _p(inst), key & 0xFFFFFFFFFFFFFFFF,
+                               _p(thr), 1 if gate else 0, _p(s), _p(de), _p(st), _p(used), _threads(nthreads))
+    assert rc == 0
+    return s, de, st, used
 
 
 def arpeggio_trace(G, h, P, T, q, r, inst, key, start, mode='arpeggio'):
