@@ -144,7 +144,8 @@ def test_features_and_training_smoke():
     assert feat.shape == (2048, feat_ld(64)) and lab.shape == (2048, 128)
     f = feat.float().cpu().numpy()
     assert np.isfinite(f).all() and (f[:, 4 * 64 + 1:] == 0).all()
-    assert np.allclose(f[:, 2 * 64:3 * 64] * 3, np.rint(f[:, 2 * 64:3 * 64] * 3))       # x_g / 3
+    levels = bf(np.array([-1, -1 / 3, 1 / 3, 1], dtype=np.float32))                     # bf16(x_g / 3)
+    assert np.isin(f[:, 2 * 64:3 * 64], levels).all()
     m = WarmStartMLP(64, seed=0)
     losses = train(m, feat, lab, epochs=6, batch=512, lr=3e-3)
     say('losses', losses)
