@@ -8,7 +8,11 @@
 // read distinct banks while the candidate's level is a broadcast.
 #include "kernels.h"
 
+#include <cstdint>
+
 namespace arp {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int gray_bits(int l, int M) {
   // QPSK: -1 -> 0, +1 -> 1.  16-QAM (MSB first): -3 00, -1 01, +1 11, +3 10
@@ -99,6 +103,140 @@ __global__ __launch_bounds__(256) void decode_select_kernel(const float* __restr
   }
 }
 
+// One-wave form for D <= 64 and Mr <= 64 (16 and 32 users): lane i keeps receive row i of H in registers (16-byte
+// loads when rows are aligned), the candidates' levels are formed into LDS (from 32-bit spin words when D % 4 == 0),
+// and each candidate's residual row sums read the levels back as broadcasts. The per-row sum runs over j in order
+// with fused multiply-adds; the row terms meet in a butterfly.
+template <int DP>
+__global__ __launch_bounds__(64) void decode_w64_kernel(const float* __restrict__ H, const float* __restrict__ y,
+                                                        const int8_t* __restrict__ xg, const int8_t* __restrict__ spins,
+                                                        int R, int P, int D, int Mr, int M, int delta, int vec4,
+                                                        int vecs,
+                                                        const int8_t* __restrict__ ltrue, int8_t* __restrict__ lhat,
+                                                        int32_t* __restrict__ biterr, float* __restrict__ dist_out) {
+  extern __shared__ float sm[];
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int C = R + 1;
+  float* Lc = sm;  // C x DP candidate levels (columns beyond D are never read)
+  const float* Hb = H + (size_t)b * Mr * D;
+  const bool row = lane < Mr;
+  float hr[DP];
+  if (vec4) {
+#pragma unroll
+    for (int q = 0; q < DP / 4; ++q) {
+      floatx4 t = {0.f, 0.f, 0.f, 0.f};
+      if (row && 4 * q < D) t = *reinterpret_cast<const floatx4*>(Hb + (size_t)lane * D + 4 * q);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) hr[4 * q + e] = t[e];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < DP; ++j) hr[j] = (row && j < D) ? Hb[(size_t)lane * D + j] : 0.f;
+  }
+  const float yi = row ? y[(size_t)b * Mr + lane] : 0.f;
+  const float lmax = M == 4 ? 1.f : 3.f;
+  const int N = delta ? P * D : (M == 4 ? D : 2 * D);
+  const bool col = lane < D;
+  if (vecs) {
+    // spins as 32-bit words: lane (g, q) forms coordinates 4q..4q+3 of candidate r0 + t G + g from word q of each copy
+    // (delta) or of each bit plane (RI); every word a lane needs is loaded before the first is used
+    const int QW = D / 4, G = 64 / QW;       // words per copy row, candidates per pass
+    const int q = lane % QW, g = lane / QW;
+    const int planes = delta ? P : (M == 4 ? 1 : 2);
+    const bool live = g < G;
+    const int xw = live ? *reinterpret_cast<const int*>(xg + (size_t)b * D + 4 * q) : 0;
+    for (int r0 = 0; r0 < C; r0 += 4 * G) {
+      int w[4][8];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int r = r0 + t * G + g;
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+          w[t][p] = (live && r < R && p < planes)
+                        ? *reinterpret_cast<const int*>(spins + ((size_t)b * R + r) * N + (size_t)p * D + 4 * q)
+                        : 0;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int r = r0 + t * G + g;
+        if (!live || r >= C) continue;
+        floatx4 lv;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int x = (int)(int8_t)(xw >> (8 * e));
+          float l;
+          if (r == R) {
+            l = (float)x;
+          } else if (delta) {
+            int d = 0;
+#pragma unroll
+            for (int p = 0; p < 8; ++p)
+              if (p < planes) d += (int)(int8_t)(w[t][p] >> (8 * e));
+            l = fminf(fmaxf((float)(x + d), -lmax), lmax);
+          } else {
+            const int s0 = (int)(int8_t)(w[t][0] >> (8 * e));
+            l = M == 4 ? (float)s0 : (float)(2 * s0 + (int)(int8_t)(w[t][1] >> (8 * e)));
+          }
+          lv[e] = l;
+        }
+        *reinterpret_cast<floatx4*>(Lc + r * DP + 4 * q) = lv;
+      }
+    }
+  } else {
+    const int x0 = col ? (int)xg[(size_t)b * D + lane] : 0;
+    for (int r = 0; r < C; ++r) {
+      float l = 0.f;
+      if (col) {
+        if (r == R) {
+          l = (float)x0;
+        } else {
+          const int8_t* s = spins + ((size_t)b * R + r) * N;
+          if (delta) {
+            int d = 0;
+            for (int p = 0; p < P; ++p) d += s[p * D + lane];
+            l = fminf(fmaxf((float)(x0 + d), -lmax), lmax);
+          } else {
+            l = M == 4 ? (float)s[lane] : (float)(2 * s[lane] + s[D + lane]);
+          }
+        }
+      }
+      if (lane < DP) Lc[r * DP + lane] = l;
+    }
+  }
+  __syncthreads();
+  int best = 0;
+  float bd = 0.f;
+  for (int r = 0; r < C; ++r) {
+    const float* lc = Lc + r * DP;
+    float sacc = 0.f;
+#pragma unroll
+    for (int j = 0; j < DP; j += 4) {
+      const floatx4 lv = *reinterpret_cast<const floatx4*>(lc + j);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (j + e < D) sacc = __builtin_fmaf(hr[j + e], lv[e], sacc);
+    }
+    const float e = yi - sacc;
+    float part = row ? e * e : 0.f;
+    for (int o = 32; o; o >>= 1) part += __shfl_xor(part, o);
+    if (dist_out && lane == 0) dist_out[(size_t)b * C + r] = part;
+    if (r == 0 || part < bd) {
+      bd = part;
+      best = r;
+    }
+  }
+  int errs = 0;
+  if (col) {
+    const int l = (int)Lc[best * DP + lane];
+    if (lhat) lhat[(size_t)b * D + lane] = (int8_t)l;
+    if (ltrue) errs = __builtin_popcount(gray_bits(l, M) ^ gray_bits(ltrue[(size_t)b * D + lane], M));
+  }
+  if (biterr && ltrue) {
+    for (int o = 32; o; o >>= 1) errs += __shfl_xor(errs, o);
+    if (lane == 0) biterr[b] = errs;
+  }
+}
+
 size_t decode_lds_bytes(int R, int D, int Mr) {
   return sizeof(float) * ((size_t)Mr * (D + 1) + Mr + (size_t)(R + 1) * D + (R + 1 > 4 ? R + 1 : 4));
 }
@@ -106,6 +244,23 @@ size_t decode_lds_bytes(int R, int D, int Mr) {
 hipError_t launch_decode_select(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R,
                                 int P, int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat,
                                 int32_t* biterr, float* dist_out, hipStream_t st) {
+  if (D <= 64 && Mr <= 64) {
+    const int DP = (D + 15) & ~15;
+    const size_t l64 = sizeof(float) * (size_t)(R + 1) * DP;
+    const int vec4 = (D % 4 == 0) && ((reinterpret_cast<uintptr_t>(H) & 15) == 0);
+    const int planes = delta ? P : (M == 4 ? 1 : 2);
+    const int vecs = (D % 4 == 0) && planes <= 8 && ((reinterpret_cast<uintptr_t>(spins) & 3) == 0) &&
+                     ((reinterpret_cast<uintptr_t>(xg) & 3) == 0);
+#define ARP_DECODE_W64(DPv)                                                                                        \
+  hipLaunchKernelGGL((decode_w64_kernel<DPv>), dim3(B), dim3(64), l64, st, H, y, xg, spins, R, P, D, Mr, M, delta,   \
+                     vec4, vecs, ltrue, lhat, biterr, dist_out)
+    if (DP == 16) ARP_DECODE_W64(16);
+    else if (DP == 32) ARP_DECODE_W64(32);
+    else if (DP == 48) ARP_DECODE_W64(48);
+    else ARP_DECODE_W64(64);
+#undef ARP_DECODE_W64
+    return hipGetLastError();
+  }
   const size_t lds = decode_lds_bytes(R, D, Mr);
   if (lds > 65536)
     hipFuncSetAttribute((const void*)decode_select_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

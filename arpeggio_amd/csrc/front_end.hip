@@ -233,15 +233,19 @@ __global__ __launch_bounds__(256) void front_end_kernel(const float* __restrict_
 
 // ---------------------------------------------------------------------------------------------------------------
 // One-wave form for D <= 64 and Mr <= 64 (16 and 32 users): lane j owns column j of A = G + sigma I in registers and
-// inverts it in place by Gauss-Jordan without pivoting (A is symmetric positive definite): step k broadcasts column
-// k through LDS and every lane applies one FMA per element, D^2 FMAs per lane in all. A^-1 is symmetric, so lane
-// i's column is row i: z_i = sum_j (A^-1)_ij c_j and (A^-1)_ii stay in lane i. h is formed as 2 (c - G x_g) from
-// the LDS-resident Gram. LDS holds H (Mp x (DP+1)) until the Gram tiles and c are formed, then G (DP x (DP+1)) in
-// the same place: 17 KiB per frame at D = 64, rows padded against bank conflicts.
+// inverts it in place with the symmetric sweep operator (A is symmetric positive definite, no pivoting): sweeping
+// pivot k maps a_kk -> -1/a_kk, a_ik -> a_ik/a_kk, a_kj -> a_kj/a_kk and a_ij -> a_ij - a_ik a_kj / a_kk, so every
+// intermediate matrix stays symmetric and, after all pivots, holds -A^-1. Symmetry means the pivot column k that
+// every lane needs is the k-th entry of each lane's own column: one all-lane LDS store publishes it and the update
+// reads it back as broadcasts, D^2 FMAs per lane in all. Lane i's column of -A^-1 is row i, so z_i and (A^-1)_ii stay
+// in lane i. h is formed as 2 (c - G x_g) from the LDS-resident Gram. LDS holds H (Mp x (DP+1)) until the Gram tiles
+// and c are formed, then G (DP x (DP+1)) in the same place: 17 KiB per frame at D = 64, rows padded against bank
+// conflicts. vec4: H rows are 16-byte aligned (D % 4 == 0 and an aligned base), so H is fetched with 16-byte loads,
+// all of a lane's loads in flight before the first LDS store.
 template <int DP>
 __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restrict__ H, const float* __restrict__ y,
                                                            const float* __restrict__ N0v, float N0s, int Mr, int D,
-                                                           int M, int K, float Q, FrontOut o) {
+                                                           int M, int K, float Q, int vec4, FrontOut o) {
   extern __shared__ float sm[];
   const int b = blockIdx.x, lane = threadIdx.x;
   const int Mp = (Mr + 3) & ~3, ld = DP + 1;
@@ -251,34 +255,66 @@ __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restri
   float* ys = sm + rows * ld; // Mp
   float* colb = ys + Mp;      // 64 words: broadcast buffer
   const float* Hb = H + (size_t)b * Mr * D;
-  for (int x = lane; x < Mp * DP; x += 64) {
-    const int r = x / DP, cc = x - r * DP;
-    Hs[r * ld + cc] = (r < Mr && cc < D) ? Hb[(size_t)r * D + cc] : 0.f;
+  if (vec4) {
+    constexpr int CPR = DP / 4;                   // 16-byte chunks per row
+    constexpr int RPP = 64 / CPR;                 // rows per pass of the wave
+    constexpr int NPASS = (64 + RPP - 1) / RPP;   // passes covering 64 rows
+    const int cr = lane % CPR, rr = lane / CPR;
+    const bool live = rr < RPP;
+    floatx4 v[NPASS];
+#pragma unroll
+    for (int t = 0; t < NPASS; ++t) {
+      const int r = t * RPP + rr;
+      v[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      if (live && r < Mr && 4 * cr < D) v[t] = *reinterpret_cast<const floatx4*>(Hb + (size_t)r * D + 4 * cr);
+    }
+#pragma unroll
+    for (int t = 0; t < NPASS; ++t) {
+      const int r = t * RPP + rr;
+      if (live && r < Mp) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Hs[r * ld + 4 * cr + e] = v[t][e];
+      }
+    }
+  } else {
+    for (int x = lane; x < Mp * DP; x += 64) {
+      const int r = x / DP, cc = x - r * DP;
+      Hs[r * ld + cc] = (r < Mr && cc < D) ? Hb[(size_t)r * D + cc] : 0.f;
+    }
   }
-  for (int x = lane; x < Mp; x += 64) ys[x] = x < Mr ? y[(size_t)b * Mr + x] : 0.f;
+  if (lane < Mp) ys[lane] = lane < Mr ? y[(size_t)b * Mr + lane] : 0.f;
   __syncthreads();
-  // Gram on the matrix cores: the upper 16x16 tiles stay in the accumulators until H is no longer needed
+  // Gram on the matrix cores: per 4-row slice of H each lane reads one operand per 16-column block, shared by the
+  // nU upper tiles; the tiles stay in the accumulators until H is no longer needed
   constexpr int nT = DP / 16, nU = nT * (nT + 1) / 2;
   floatx4 acc[nU];
+#pragma unroll
+  for (int u = 0; u < nU; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
   {
     const int li = lane & 15, lk = lane >> 4;
-    int u = 0;
+#pragma unroll 2
+    for (int kk = 0; kk < Mp; kk += 4) {
+      float op[nT];
 #pragma unroll
-    for (int I = 0; I < nT; ++I)
+      for (int I = 0; I < nT; ++I) op[I] = Hs[(kk + lk) * ld + 16 * I + li];
+      int u = 0;
 #pragma unroll
-      for (int J = I; J < nT; ++J, ++u) {
-        floatx4 t = {0.f, 0.f, 0.f, 0.f};
-        for (int kk = 0; kk < Mp; kk += 4) {
-          const float a0 = Hs[(kk + lk) * ld + 16 * I + li];
-          const float b0 = Hs[(kk + lk) * ld + 16 * J + li];
-          t = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, t, 0, 0, 0);
-        }
-        acc[u] = t;
-      }
+      for (int I = 0; I < nT; ++I)
+#pragma unroll
+        for (int J = I; J < nT; ++J, ++u) acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(op[I], op[J], acc[u], 0, 0, 0);
+    }
   }
-  // c_j = (H^T y)_j
-  float c = 0.f;
-  for (int k = 0; k < Mr; ++k) c += Hs[k * ld + lane] * ys[k];
+  // c_j = (H^T y)_j, four partial sums
+  float c;
+  {
+    float c4[4] = {0.f, 0.f, 0.f, 0.f};
+    const int kq = Mp & ~15;
+    for (int k = 0; k < kq; k += 16)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) c4[e & 3] += Hs[(k + e) * ld + lane] * ys[k + e];
+    for (int k = kq; k < Mp; ++k) c4[0] += Hs[k * ld + lane] * ys[k];
+    c = (c4[0] + c4[1]) + (c4[2] + c4[3]);
+  }
   __syncthreads();
   {
     const int li = lane & 15, lk = lane >> 4;
@@ -305,32 +341,31 @@ __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restri
     const float g = lane < DP ? Gs[i * ld + lane] : 0.f;
     a[i] = valid ? (i == lane ? g + sigma : (i < D ? g : 0.f)) : (i == lane ? 1.f : 0.f);
   }
-  // Gauss-Jordan, in place: after step k, column k holds -A_ik / A_kk (i != k) and 1 / A_kk; other columns are
-  // reduced by the scaled pivot row. Lane k takes rk = 1 + p so that the shared update gives its -f p. Column k is
-  // broadcast through LDS (lane k stores it, every lane reads it back at one address per word), so the update runs
-  // on vector registers; both loops are fully unrolled so that a[k] and a[i] are fixed registers.
+  // Sweep, in place. Step k: every lane publishes a_kj (its own a[k]); lane j != k forms r = a_kj / a_kk and
+  // subtracts a_ik r from a_ij, then keeps r as a_kj; lane k takes r = 1 - 1/a_kk, which turns the same update into
+  // a_ik -> a_ik / a_kk, and keeps -1/a_kk. The broadcast buffer alternates between two halves so that a step's
+  // store never meets the previous step's reads; both loops are fully unrolled so that a[k] and a[i] are fixed
+  // registers.
 #pragma clang loop unroll(full)
   for (int k = 0; k < DP; ++k) {
-    if (lane == k) {
-#pragma unroll
-      for (int i = 0; i < DP; i += 4)
-        *reinterpret_cast<floatx4*>(colb + i) = floatx4{a[i], a[i + 1], a[i + 2], a[i + 3]};
-    }
+    float* cb = colb + (k & 1) * 64;
+    cb[lane] = a[k];
     __syncthreads();
-    const float p = 1.f / colb[k];
+    const float p = __builtin_amdgcn_rcpf(cb[k]);
     const bool is_k = lane == k;
-    const float rk = is_k ? 1.f + p : a[k] * p;
+    const float rk = is_k ? 1.f - p : a[k] * p;
 #pragma clang loop unroll(full)
     for (int i = 0; i < DP; i += 4) {
-      const floatx4 f = *reinterpret_cast<const floatx4*>(colb + i);
+      const floatx4 f = *reinterpret_cast<const floatx4*>(cb + i);
 #pragma unroll
       for (int v = 0; v < 4; ++v)
         if (i + v != k) a[i + v] = __builtin_fmaf(-f[v], rk, a[i + v]);
     }
-    a[k] = is_k ? p : rk;
-    __syncthreads();
+    a[k] = is_k ? -p : rk;
   }
-  // z_i = sum_j (A^-1)_ij c_j, d_i = (A^-1)_ii; c and, below, x_g are broadcast through LDS the same way
+  __syncthreads();
+  // z_i = sum_j (A^-1)_ij c_j, d_i = (A^-1)_ii (the array holds -A^-1); c and, below, x_g are broadcast through LDS
+  // the same way
   colb[lane] = c;
   __syncthreads();
   float z = 0.f, d = 0.f;
@@ -339,8 +374,8 @@ __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restri
     const floatx4 cj = *reinterpret_cast<const floatx4*>(colb + j);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      z = __builtin_fmaf(a[j + v], cj[v], z);
-      d = j + v == lane ? a[j + v] : d;
+      z = __builtin_fmaf(-a[j + v], cj[v], z);
+      d = j + v == lane ? -a[j + v] : d;
     }
   }
   __syncthreads();
@@ -418,8 +453,9 @@ template <int DP>
 static void launch_w64(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M, int K,
                        float Q, const FrontOut& o, hipStream_t st) {
   const int Mp = (Mr + 3) & ~3;
-  const size_t lds = sizeof(float) * ((size_t)(Mp > DP ? Mp : DP) * (DP + 1) + Mp + 64);
-  hipLaunchKernelGGL((front_end_w64_kernel<DP>), dim3(B), dim3(64), lds, st, H, y, N0v, N0s, Mr, D, M, K, Q, o);
+  const size_t lds = sizeof(float) * ((size_t)(Mp > DP ? Mp : DP) * (DP + 1) + Mp + 128);
+  const int vec4 = (D % 4 == 0) && ((reinterpret_cast<uintptr_t>(H) & 15) == 0);
+  hipLaunchKernelGGL((front_end_w64_kernel<DP>), dim3(B), dim3(64), lds, st, H, y, N0v, N0s, Mr, D, M, K, Q, vec4, o);
 }
 
 hipError_t launch_front_end(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M,

@@ -124,9 +124,15 @@ class DetectPipeline:
             raise ValueError('policy must be arpeggio or chord')
         self.Q = float(Q)
 
-    def run(self, H, y, N0, ltrue=None, stream=None):
-        """Launches front end, anneal and decode on one stream; returns (lhat, biterr or None) device tensors."""
+    def run(self, H, y, N0, ltrue=None, stream=None, events=None):
+        """Launches front end, anneal and decode on one stream; returns (lhat, biterr or None) device tensors.
+        events: optional (start, end) torch.cuda.Event pair recorded on the same stream around the anneal launch."""
         L, s = self.lib, _lib.stream_ptr(stream)
+        ts = stream if stream is not None else torch.cuda.current_stream()
+
+        def mark(i):
+            if events is not None:
+                events[i].record(ts)
         B, Mr, D, M, R, P = self.B, self.Mr, self.D, self.M, self.R, self.P
         n0t, n0s = _n0_arg(N0, B)
         P_ = _lib.ptr
@@ -135,25 +141,31 @@ class DetectPipeline:
                                        P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None,
                                        None, None, P_(self.feat), self.feat.shape[1], s), 'front_end')
             self.model.thresholds(self.feat, self.inv_temp, out=self.thr)
+            mark(0)
             _lib.check(L.arp_anneal_arpeggio_ws(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D,
                                                 B, P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
                                                 P_(self.thr), 1 if self.gate else 0, P_(self.spins), P_(self.de),
                                                 P_(self.st), P_(self.used), s), 'arpeggio_ws')
+            mark(1)
         elif self.policy == 'arpeggio':
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
                                        P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None,
                                        None, None, None, 0, s), 'front_end')
+            mark(0)
             _lib.check(L.arp_anneal_arpeggio(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D, B,
                                              P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
                                              0 if self.start == 'rand' else 1, 0 if self.mode == 'arpeggio' else 1,
                                              P_(self.spins), P_(self.de), P_(self.st), s), 'arpeggio')
+            mark(1)
         else:
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg), None,
                                        None, P_(self.scale), None, None, P_(self.J), P_(self.hN), None, None, None,
                                        None, None, 0, s), 'front_end')
+            mark(0)
             _lib.check(L.arp_anneal_chord(P_(self.J), P_(self.hN), P_(self.hN), P_(self.T), P_(self.c), self.q, self.S,
                                           P, R, P * D, B, self.key & MASK, (self.key >> 32) & MASK, self.inst0, 0,
                                           P_(self.e), P_(self.spins), None, s), 'chord')
+            mark(1)
         _lib.check(L.arp_decode_select(P_(H), P_(y), P_(self.xg), P_(self.spins), B, R, P, D, Mr, M, 1, P_(ltrue),
                                        P_(self.lhat), P_(self.biterr) if ltrue is not None else None, None, s),
                    'decode_select')

@@ -1,17 +1,27 @@
 #!/usr/bin/env python3
-"""Detections/s of the one-stage delta-Ising detector on one MI355X: 32 users x 32 rx, 16-QAM, i.i.d. Rayleigh,
-16 anneals per frame, 65,536 frames per step.
+"""Detections/s at fixed BER on one MI355X, learned-start vs plain annealer: 32 users x 32 rx, 16-QAM, i.i.d.
+Rayleigh, 16 anneals per frame, 65,536 frames per step, one-stage dDI (K = 1).
 
-A step is one pass of the kernel path over the batch, with H, y and the true levels already in HBM:
-front end (LMMSE + dDI Ising build + fixed point) -> fixed-point Arpeggio anneal (first-accept, 64 passes) ->
-decode/select -> bit errors. `value` = frames / s over the K timed steps.
+A step is one pass of the kernel path over the batch with H, y and the true levels already in HBM:
+front end (LMMSE + dDI Ising build + fixed point [+ warm-start features]) -> [MLP -> start thresholds] ->
+fixed-point Arpeggio anneal (first accept) -> decode/select -> bit errors.
+
+Procedure (fixed-BER comparison):
+  1. plain = random start, 64 passes (32 sweeps x 2 copies), T = 0.1, q = 0 (the reference's 32-user pick
+     abl_far_32_0.1_0.1_0_rand_*); the SNR where plain reaches BER 1e-3 is found on calibration frames
+     (log-BER interpolated between bracketing SNRs; --snr skips this);
+  2. on the measurement frames at that SNR, the learned start runs at each budget of the ladder; the chosen
+     budget is the smallest whose paired-bootstrap BER ratio to plain (95 % CI) reaches down to 1.0;
+  3. both pipelines are timed over K steps; value = learned-start detections/s at the chosen budget,
+     plain.value and speedup beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--snr DB] [--frames B] [--no-cpu] [--with-chord]
 
-Extra fields: roofline (the anneal kernel, timed with HIP events on its own stream), cpu_baseline (the C oracle
-of the same chain on a bounded sample, on this host's cores), ber over the timed steps."""
+Extra fields: roofline (the timed step's anneal kernel, HIP events on its own stream), cpu_baseline (the C oracle
+of the plain chain on a bounded sample, on this host's cores)."""
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -22,7 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = 'one-stage dDI detection, 32x32 16-QAM i.i.d. Rayleigh, Arpeggio fixed point, R=16, L=64 passes'
+NT, M = 32, 16
+METRIC = 'detections/s at fixed BER (32u 16-QAM), learned-start vs plain annealer'
 
 
 def parse():
@@ -31,15 +42,24 @@ def parse():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--frames', type=int, default=65536)
-    ap.add_argument('--snr', type=float, default=24.0)
+    ap.add_argument('--calib-frames', type=int, default=32768)
+    ap.add_argument('--snr', type=float, default=None, help='skip the calibration and measure at this SNR (dB)')
+    ap.add_argument('--target-ber', type=float, default=1e-3)
     ap.add_argument('--copies', type=int, default=16)
-    ap.add_argument('--passes', type=int, default=64)
+    ap.add_argument('--passes', type=int, default=64, help='plain annealer pass budget')
+    ap.add_argument('--ladder', default='16,32,48,64,128,256', help='learned-start pass budgets tried, ascending')
+    ap.add_argument('--weights', default=None)
     ap.add_argument('--seed', type=int, default=1)
+    ap.add_argument('--plain-only', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-frames', type=int, default=4096)
     ap.add_argument('--with-chord', action='store_true')
     ap.add_argument('--copy-probe', action='store_true', help='run only a plain device copy of known size (PMC scale)')
     return ap.parse_args()
+
+
+def log(*a):
+    print('#', *a, file=sys.stderr, flush=True)
 
 
 def copy_probe():
@@ -53,12 +73,74 @@ def copy_probe():
     print(json.dumps(dict(copy_probe_bytes=n, launches=3)))
 
 
+def key_of(seed):
+    return (seed * 0x9E3779B97F4A7C15 + 0xA513) & 0xFFFFFFFFFFFFFFFF
+
+
+def make_pipe(B, args, L, start, model=None):
+    from arpeggio_amd.detector import DetectPipeline
+    return DetectPipeline(B, 2 * NT, 2 * NT, M, args.copies, policy='arpeggio', cycles=L, T0=0.1, T1=0.1, q=0.0,
+                          start=start, key=key_of(args.seed), inst0=0, model=model)
+
+
+def frame_errors(pipe, b):
+    import torch
+    lt = b['l'].to(torch.int8).contiguous()
+    _, be = pipe.run(b['H'], b['y'], b['N0'], lt)
+    return be.cpu().numpy().astype(np.int64)
+
+
+def calibrate(args):
+    """SNR (dB) where the plain annealer's BER crosses the target, on calibration frames (draw index 1)."""
+    from arpeggio_amd import frames
+    Bc = args.calib_frames
+    pipe = make_pipe(Bc, args, args.passes, 'rand')
+    pts = []
+    for snr in (22.0, 23.0, 24.0, 25.0, 26.0):
+        b = frames.draw(args.seed, snr, 1, Bc, NT, M, dev='cuda')
+        ber = frame_errors(pipe, b).sum() / (Bc * 4 * NT)
+        log('calibration: plain BER %.3e at %.1f dB' % (ber, snr))
+        pts.append((snr, ber))
+        if ber <= args.target_ber:
+            break
+    hi = [p for p in pts if p[1] > args.target_ber]
+    lo = [p for p in pts if 0 < p[1] <= args.target_ber]
+    if not hi or not lo:
+        raise RuntimeError('calibration did not bracket BER %g: %s' % (args.target_ber, pts))
+    (s0, b0), (s1, b1) = hi[-1], lo[0]
+    t = (math.log10(b0) - math.log10(args.target_ber)) / (math.log10(b0) - math.log10(b1))
+    return round(s0 + t * (s1 - s0), 2), pts
+
+
+def time_pipe(pipe, b, steps, warmup):
+    """K timed steps after W warm-up steps; barrier = synchronize on both sides. Returns det/s, ms/step, mean anneal
+    kernel ms (HIP events on the launch stream), BER over the timed steps."""
+    import torch
+    lt = b['l'].to(torch.int8).contiguous()
+    H, y, N0 = b['H'], b['y'], b['N0']
+    for _ in range(warmup):
+        pipe.run(H, y, N0, lt)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    errs = torch.zeros((), dtype=torch.int64, device='cuda')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        _, be = pipe.run(H, y, N0, lt, events=ev[i])
+        errs += be.sum()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    B = pipe.B
+    return dict(value=B * steps / dt, ms_per_step=dt / steps * 1e3,
+                anneal_ms=float(np.mean([a.elapsed_time(c) for a, c in ev])),
+                ber=float(errs.item()) / (B * 4 * NT * steps))
+
+
 def cpu_baseline(b, pipe, B_cpu, args):
-    """The oracle's chain on the first B_cpu frames on the host cores: numpy float64 front end, the C fixed-point
-    Arpeggio (threads = min(16, cores)) on the GPU front end's integers, numpy decode. Wall time -> frames/s."""
+    """The oracle's plain chain on the first B_cpu frames on the host cores: numpy float64 front end, the C
+    fixed-point Arpeggio (threads = min(16, cores)) on the GPU front end's integers, numpy decode. Wall time ->
+    frames/s."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle
-    import torch
     cores = min(16, os.cpu_count() or 1)
     H = b['H'][:B_cpu].cpu().numpy()
     y = b['y'][:B_cpu].cpu().numpy()
@@ -68,19 +150,47 @@ def cpu_baseline(b, pipe, B_cpu, args):
     inst = np.arange(B_cpu)
     oracle.lib()
     t0 = time.perf_counter()
-    z, Ainv, _ = oracle.mmse(H, y, b['N0'], 16)
-    xg = oracle.mmse_hard(H, y, b['N0'], 16, z, Ainv)
+    z, Ainv, _ = oracle.mmse(H, y, b['N0'], M)
+    xg = oracle.mmse_hard(H, y, b['N0'], M, z, Ainv)
     oracle.ising_delta_at(H, y, xg, 1)
     s, de, st = oracle.arpeggio_fx(G, h, 2, args.copies, T, pipe.q, inst, pipe.key, 'rand', nthreads=cores)
-    Lc = oracle.spins_to_levels_delta(s, xg, 1, 16)
+    Lc = oracle.spins_to_levels_delta(s, xg, 1, M)
     oracle.select(H, y, Lc, xg)
     dt = time.perf_counter() - t0
     gs = pipe.spins[:B_cpu].cpu().numpy()
     agree = bool((gs == s).all())
     return dict(value=B_cpu / dt, unit='detections/s', cores=cores, kind='port',
-                sample='first %d frames of the step: float64 numpy front end, C oracle Arpeggio (%d threads) on the '
-                       'same integer inputs, numpy decode/select; %.1f s wall; anneal spins equal the GPU: %s'
-                       % (B_cpu, cores, dt, agree))
+                sample='first %d frames of the plain step: float64 numpy front end, C oracle Arpeggio (%d threads, '
+                       '%d passes, R=%d) on the same integer inputs, numpy decode/select; %.1f s wall; anneal spins '
+                       'equal the GPU: %s' % (B_cpu, cores, args.passes, args.copies, dt, agree))
+
+
+def roofline(pipe, t, B, L, R, learned):
+    """Dominant kernel of the timed step = the anneal. Algorithmic HBM bytes per launch: G (int32, D x D) + h + inst
+    per frame and T per pass in [+ start thresholds P D int32 per frame]; spins (R P D int8), best energy (int64),
+    (flips, tests) (2 int64) [+ used (int8)] per anneal out."""
+    D, P = 2 * NT, 2
+    hbm = B * (4 * D * D + 4 * D + 4) + 4 * L + B * R * (P * D + 8 + 16)
+    if learned:
+        hbm += B * 4 * P * D + B * R
+    ms = t['anneal_ms']
+    achieved = hbm / (ms * 1e-3) / 1e9
+    st = pipe.st.double()
+    flips = float(st[..., 0].mean())
+    lds = B * R * 4 * D * (flips + D)
+    traffic = None
+    kern = 'arpeggio_fx_kernel<2,1>'
+    tp = os.path.join(ROOT, 'profiles', 'arpeggio_anneal_traffic.json')
+    if os.path.exists(tp):
+        for rec in json.load(open(tp)).get('launches', []):
+            if (rec.get('frames'), rec.get('passes'), rec.get('copies'), rec.get('learned')) == (B, L, R, learned):
+                traffic = rec.get('hbm_bytes_per_launch')
+    return dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
+                traffic=traffic, kernel=kern + (' (learned start)' if learned else ''), algorithmic_bytes=hbm,
+                launch_ms=ms, flips_per_anneal=flips,
+                lds=dict(algorithmic_bytes=lds, achieved_GBs=lds / (ms * 1e-3) / 1e9),
+                note='the anneal is an on-chip loop bound by scalar/vector integer issue (Philox4x32-10, the 64-bit '
+                     'decision, the ballot scan), not by HBM or the matrix cores; bytes are the algorithmic ones')
 
 
 def main():
@@ -93,95 +203,80 @@ def main():
         copy_probe()
         return
     from arpeggio_amd import frames
-    from arpeggio_amd.detector import DetectPipeline
-    B, nt, M, R, L = args.frames, 32, 16, args.copies, args.passes
-    print('# drawing %d frames (CPU generator)' % B, file=sys.stderr, flush=True)
-    b = frames.draw(args.seed, args.snr, 0, B, nt, M, dev='cuda')
-    ltrue = b['l'].to(torch.int8).contiguous()
-    H, y = b['H'].contiguous(), b['y'].contiguous()
-    key = (args.seed * 0x9E3779B97F4A7C15 + 0xA513) & 0xFFFFFFFFFFFFFFFF
-    pipe = DetectPipeline(B, 2 * nt, 2 * nt, M, R, policy='arpeggio', cycles=L, T0=0.1, T1=0.1, q=0.0, start='rand',
-                          key=key, inst0=0)
-    stream = torch.cuda.current_stream()
-    # the anneal kernel's own duration: events on the stream it is launched on
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    L_ = pipe.lib
-    from arpeggio_amd import _lib
-
-    def step(i=None):
-        P_ = _lib.ptr
-        s = _lib.stream_ptr(stream)
-        _lib.check(L_.arp_front_end(P_(H), P_(y), None, float(b['N0']), B, 2 * nt, 2 * nt, M, 1, pipe.Q, P_(pipe.xg),
-                                    P_(pipe.Gi), P_(pipe.hi), P_(pipe.scale), None, None, None, None, None, None, None, None,
-                                    None, 0, s),
-                   'front_end')
-        if i is not None:
-            ev[i][0].record(stream)
-        _lib.check(L_.arp_anneal_arpeggio(P_(pipe.Gi), P_(pipe.hi), P_(pipe.T), pipe.T.numel(), pipe.q, 2, R, 2 * nt, B,
-                                          P_(pipe.inst), key & 0xFFFFFFFF, key >> 32, 0, 0, P_(pipe.spins), P_(pipe.de),
-                                          P_(pipe.st), s), 'arpeggio')
-        if i is not None:
-            ev[i][1].record(stream)
-        _lib.check(L_.arp_decode_select(P_(H), P_(y), P_(pipe.xg), P_(pipe.spins), B, R, 2, 2 * nt, 2 * nt, M, 1,
-                                        P_(ltrue), P_(pipe.lhat), P_(pipe.biterr), None, s), 'decode')
-
-    print('# warmup %d' % args.warmup, file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    errs = torch.zeros((), dtype=torch.int64, device='cuda')
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(i)
-        errs += pipe.biterr.sum()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = dt / args.steps * 1e3
-    value = B * args.steps / dt
-    ann_ms = float(np.mean([a.elapsed_time(c) for a, c in ev]))
-    nbits = B * 2 * nt * 2 * args.steps
-    ber = float(errs.item()) / nbits
-    st = pipe.st.double()
-    flips, tests = float(st[..., 0].mean()), float(st[..., 1].mean())
-    D = 2 * nt
-    # algorithmic HBM bytes of one anneal launch: G (int32) + h + inst + T in; spins, de, (flips, tests) out
-    hbm_bytes = B * (4 * D * D + 4 * D + 4) + 4 * L + B * R * (2 * D + 8 + 16)
-    achieved = hbm_bytes / (ann_ms * 1e-3) / 1e9
-    # algorithmic LDS bytes: one int32 Gram column per committed flip plus the D x D initial-field sweep per anneal
-    lds_bytes = B * R * 4 * D * (flips + D)
-    traffic = None
-    tp = os.path.join(ROOT, 'profiles', 'arpeggio_anneal_traffic.json')
-    if os.path.exists(tp):
-        t = json.load(open(tp))
-        if t.get('frames') == B and t.get('passes') == L and t.get('copies') == R:
-            traffic = t.get('hbm_bytes_per_launch')
-    out = dict(metric='detections/s at fixed BER (32u 16-QAM), learned-start vs plain annealer', value=value,
-               unit='detections/s', n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=ms_step,
-               higher_is_better=True, scaling='weak', vs_baseline=None, dtype='int32', data='synthetic',
-               config=dict(workload=WORKLOAD, frames=B, users=nt, rx=nt, M=M, copies=R, passes=L, policy='arpeggio',
-                           start='rand', snr_db=args.snr, T0=0.1, T1=0.1, q=0.0),
-               ber=ber, anneal=dict(ms=ann_ms, flips_per_anneal=flips, tests_per_anneal=tests,
-                                    share_of_step=ann_ms / ms_step),
-               roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
-                             traffic=traffic, kernel='arpeggio_fx_kernel<2,1>',
-                             lds=dict(algorithmic_bytes=lds_bytes, achieved_GBs=lds_bytes / (ann_ms * 1e-3) / 1e9),
-                             note='the anneal is bound by integer VALU issue (Philox4x32-10 and the 64-bit decision), '
-                                  'not by HBM or the matrix cores'))
+    from arpeggio_amd.stats import boot_ratio
+    B, R, L = args.frames, args.copies, args.passes
+    if args.snr is None:
+        snr, calib = calibrate(args)
+    else:
+        snr, calib = args.snr, None
+    log('measuring at %.2f dB; drawing %d frames (CPU generator)' % (snr, B))
+    b = frames.draw(args.seed, snr, 0, B, NT, M, dev='cuda')
+    plain = make_pipe(B, args, L, 'rand')
+    be_plain = frame_errors(plain, b)
+    nbits = B * 4 * NT
+    out = dict(metric=METRIC, unit='detections/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
+               higher_is_better=True, scaling='weak', vs_baseline=None, dtype='int32', data='synthetic')
+    ladder, chosen, model = [], None, None
+    if not args.plain_only:
+        from arpeggio_amd.learned import DEFAULT_WEIGHTS, WarmStartMLP
+        model = WarmStartMLP.load(args.weights or DEFAULT_WEIGHTS)
+        for Ll in [int(x) for x in args.ladder.split(',')]:
+            pl = make_pipe(B, args, Ll, 'learned', model)
+            be = frame_errors(pl, b)
+            r, lo, hi = boot_ratio(be, be_plain)
+            rec = dict(passes=Ll, ber=float(be.sum()) / nbits, ratio=r, ci=[lo, hi],
+                       kept_learned=float(pl.used.double().mean()))
+            log('learned start, %d passes: BER %.3e, ratio to plain %.3f [%.3f, %.3f]' % (Ll, rec['ber'], r, lo, hi))
+            ladder.append(rec)
+            del pl
+            if lo <= 1.0:
+                chosen = Ll
+                break
+    log('timing plain (%d passes)' % L)
+    tp = time_pipe(plain, b, args.steps, args.warmup)
+    plain_rec = dict(value=tp['value'], ms_per_step=tp['ms_per_step'], passes=L, start='rand',
+                     ber=float(be_plain.sum()) / nbits, anneal_ms=tp['anneal_ms'])
+    if chosen is not None:
+        log('timing learned start (%d passes)' % chosen)
+        lp = make_pipe(B, args, chosen, 'learned', model)
+        tl = time_pipe(lp, b, args.steps, args.warmup)
+        out.update(value=tl['value'], ms_per_step=tl['ms_per_step'])
+        out['learned'] = dict(passes=chosen, ms_per_step=tl['ms_per_step'], ber=tl['ber'], anneal_ms=tl['anneal_ms'],
+                              ber_matched=True)
+        out['speedup_vs_plain'] = tl['value'] / tp['value']
+        out['roofline'] = roofline(lp, tl, B, chosen, R, True)
+        timed = 'learned'
+    else:
+        # no budget matched (or --plain-only): the plain annealer is the measured step
+        out.update(value=tp['value'], ms_per_step=tp['ms_per_step'])
+        if not args.plain_only:
+            out['learned'] = dict(passes=None, ber_matched=False)
+        out['roofline'] = roofline(plain, tp, B, L, R, False)
+        timed = 'plain'
+    out['plain'] = plain_rec
+    out['ladder'] = ladder
+    out['config'] = dict(workload='one-stage dDI detection, 32x32 16-QAM i.i.d. Rayleigh, %d frames/step, R=%d '
+                                  'anneals/frame; %s step timed' % (B, R, timed),
+                         frames=B, users=NT, rx=NT, M=M, copies=R, plain_passes=L, learned_passes=chosen,
+                         policy='arpeggio', snr_db=snr, target_ber=args.target_ber, calibration=calib,
+                         T0=0.1, T1=0.1, q=0.0)
     if not args.no_cpu:
-        print('# cpu baseline on %d frames' % args.cpu_frames, file=sys.stderr, flush=True)
-        out['cpu_baseline'] = cpu_baseline(b, pipe, min(args.cpu_frames, B), args)
+        log('cpu baseline on %d frames' % args.cpu_frames)
+        out['cpu_baseline'] = cpu_baseline(b, plain, min(args.cpu_frames, B), args)
     if args.with_chord:
-        cp = DetectPipeline(B, 2 * nt, 2 * nt, M, R, policy='chord', cycles=256, T0=0.3, T1=0.2, q=0.4, kappa=0.3,
-                            key=key, inst0=0)
-        cp.run(H, y, b['N0'], ltrue)
+        from arpeggio_amd.detector import DetectPipeline
+        lt = b['l'].to(torch.int8).contiguous()
+        cp = DetectPipeline(B, 2 * NT, 2 * NT, M, R, policy='chord', cycles=256, T0=0.3, T1=0.2, q=0.4, kappa=0.3,
+                            key=key_of(args.seed), inst0=0)
+        cp.run(b['H'], b['y'], b['N0'], lt)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         ce = torch.zeros((), dtype=torch.int64, device='cuda')
         for _ in range(3):
-            cp.run(H, y, b['N0'], ltrue)
+            cp.run(b['H'], b['y'], b['N0'], lt)
             ce += cp.biterr.sum()
         torch.cuda.synchronize()
-        out['chord'] = dict(value=3 * B / (time.perf_counter() - t1), S=256, ber=float(ce.item()) / (3 * B * 128))
+        out['chord'] = dict(value=3 * B / (time.perf_counter() - t1), S=256, ber=float(ce.item()) / (3 * nbits))
     print(json.dumps(out), flush=True)
 
 

@@ -371,35 +371,43 @@ def energy0(s, h0, J):
 
 
 # ---------------------------------------------------------------- statistics (a56an.py:74-106)
+def _ratio(num, den):
+    if den > 0:
+        return num / den
+    return float('inf') if num > 0 else 1.0
+
+
 def boot_ratio(a, b, B=4000, seed=12345):
-    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
-    U = len(a)
+    """Paired cluster bootstrap of sum(a) / sum(b) over uses (a56an.py:74-95). Only uses with an error in either
+    vector can change a sum, so each replicate draws how many of them it holds, Binomial(U, n_err / U), and then that
+    many uniform picks among them. The generator is consumed in that order, replicate by replicate, which is what
+    makes the result reproducible against the reference's published outputs. Returns (ratio, lo, hi): the 2.5 % and
+    97.5 % order statistics of the replicates."""
+    ea, eb = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    U = ea.shape[0]
     if U == 0:
         return None, None, None
-    nz = np.nonzero((a > 0) | (b > 0))[0]
-    sa, sb = a.sum(), b.sum()
-    r0 = sa / sb if sb > 0 else (float('inf') if sa > 0 else 1.0)
-    if len(nz) == 0:
+    err = np.flatnonzero((ea > 0) | (eb > 0))
+    whole = _ratio(ea.sum(), eb.sum())
+    if err.size == 0:
         return 1.0, 1.0, 1.0
-    rng = np.random.default_rng(seed)
-    x, y = a[nz], b[nz]
-    rs = np.empty(B)
-    for i in range(B):
-        n = rng.binomial(U, len(nz) / U)
-        w = np.bincount(rng.integers(0, len(nz), n), minlength=len(nz))
-        ya = float((w * x).sum())
-        yb = float((w * y).sum())
-        rs[i] = ya / yb if yb > 0 else (float('inf') if ya > 0 else 1.0)
-    rs.sort()
-    return float(r0), float(rs[int(0.025 * (B - 1))]), float(rs[int(math.ceil(0.975 * (B - 1)))])
+    gen = np.random.default_rng(seed)
+    ca, cb = ea[err], eb[err]
+    reps = []
+    for _ in range(B):
+        k = gen.binomial(U, err.size / U)
+        cnt = np.bincount(gen.integers(0, err.size, k), minlength=err.size)
+        reps.append(_ratio(float((cnt * ca).sum()), float((cnt * cb).sum())))
+    reps.sort()
+    return float(whole), float(reps[int(0.025 * (B - 1))]), float(reps[int(math.ceil(0.975 * (B - 1)))])
 
 
 def mcnemar(a, b):
-    a, b = np.asarray(a), np.asarray(b)
-    x, y = int(((a > 0) & (b == 0)).sum()), int(((a == 0) & (b > 0)).sum())
-    n = x + y
+    """Exact two-sided McNemar test on per-use error indicators (a56an.py:98-106): (p, only-a, only-b)."""
+    fa, fb = np.asarray(a) > 0, np.asarray(b) > 0
+    only_a, only_b = int((fa & ~fb).sum()), int((fb & ~fa).sum())
+    n = only_a + only_b
     if n == 0:
-        return 1.0, x, y
-    k = min(x, y)
-    p = sum(math.comb(n, i) for i in range(k + 1)) / 2 ** n
-    return min(1.0, 2 * p), x, y
+        return 1.0, only_a, only_b
+    tail = sum(math.comb(n, i) for i in range(min(only_a, only_b) + 1))
+    return min(1.0, 2 * tail / 2 ** n), only_a, only_b
