@@ -13,6 +13,14 @@ _i32, _u32, _u64 = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64
 _l, _f = ctypes.c_long, ctypes.c_float
 
 # name -> argtypes; every entry point returns int (0 = success)
+class BatchClass(ctypes.Structure):
+    """arp_batch_class: one size class of a mixed batch (device pointers and sizes)."""
+    _fields_ = [('Gt', ctypes.c_void_p), ('h', ctypes.c_void_p), ('inst', ctypes.c_void_p), ('T', ctypes.c_void_p),
+                ('out_s', ctypes.c_void_p), ('out_de', ctypes.c_void_p), ('out_st', ctypes.c_void_p),
+                ('D', ctypes.c_int), ('B', ctypes.c_int), ('L', ctypes.c_int), ('q', ctypes.c_int),
+                ('start', ctypes.c_int), ('tile', ctypes.c_int)]
+
+
 SIGNATURES = {
     'arp_philox_kat': [_P, _P, _i32, _P],
     'arp_anneal_arpeggio': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _i32, _P, _P, _P,
@@ -23,6 +31,7 @@ SIGNATURES = {
                       _P, _P, _P, _P, _P, _P, _P, _i32, _P],
     'arp_anneal_arpeggio_ws': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _P, _i32, _P, _P, _P, _P,
                                _P],
+    'arp_anneal_arpeggio_batched': [_P, _i32, _i32, _u32, _u32, _P, _i32, _i32, _P],
     'arp_mlp_gemm': [_P, _l, _l, _P, _l, _l, _i32, _i32, _i32, _P, _i32, _P, _P, _P, _l, _P],
     'arp_mlp_gelu_bwd': [_P, _P, _l, _P, _P, _P],
     'arp_mlp_bce': [_P, _l, _P, _i32, _i32, _P, _P, _P, _P],

@@ -69,10 +69,10 @@ __device__ __forceinline__ long long start_energy(const int32_t* __restrict__ hb
   return e;
 }
 
-template <int P, int CPL>
+template <int P, int CPL, int MODE>
 __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb,
                                                 const int32_t* __restrict__ T, int L, int q, int D, uint32_t r,
-                                                uint32_t inst, uint32_t k0, uint32_t k1, int start, int mode,
+                                                uint32_t inst, uint32_t k0, uint32_t k1, int start,
                                                 int8_t* __restrict__ out_s, long long* __restrict__ out_de,
                                                 long long* __restrict__ out_st, size_t t,
                                                 const uint32_t* __restrict__ thr, int gate,
@@ -143,7 +143,9 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
     if (used && lane == 0) used[t] = take ? 1 : 0;
   }
   long long de4 = 0, best4 = 1LL << 60;   // de / 4
-  int flips = 0, last = 0;
+  int32_t fc[CPL];                        // this lane's flip count
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) fc[c] = 0;
   uint32_t rw[P][CPL][4];
   for (int kb = 0; kb < L; kb += 4 * P) {
     const int n4 = kb / (4 * P);
@@ -170,17 +172,21 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
         const int32_t Tk = Tb[nm * P + p];
         const uint32_t A = 4u * (uint32_t)Tk;
         const int32_t Bk = q + 2 * Tk;
-        int32_t tq[CPL];
+        // Per pass: tq = the decision bound (lanes past D never accept), sg = 0 / -1 for spin +1 / -1 of copy p at
+        // the start of the pass. Within a pass a lane flips at most once and is then behind the scan, so the lanes
+        // still to be tested keep their start-of-pass spins and q = s w + 2 G_jj = (w ^ sg) - sg + dj2.
+        int32_t tq[CPL], sg[CPL];
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) tq[c] = accept_threshold(rw[p][c][nm], A) - Bk;
-        if (mode == 0) {
+        for (int c = 0; c < CPL; ++c) {
+          tq[c] = valid[c] ? accept_threshold(rw[p][c][nm], A) - Bk : INT32_MIN;
+          sg[c] = ((Sm[p][c] >> lane) & 1ull) ? 0 : -1;
+        }
+        if (MODE == 0) {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
-            uint64_t rem = vmask[c];
-            while (true) {
-              const int32_t qv = (((Sm[p][c] >> lane) & 1ull) ? w[c] : -w[c]) + dj2[c];
-              const uint64_t m = ballot(qv < tq[c]) & rem;
-              if (m == 0) break;
+            int32_t qv = ((w[c] ^ sg[c]) - sg[c]) + dj2[c];
+            uint64_t m = ballot(qv < tq[c]);
+            while (m) {
               const int l = __builtin_ctzll(m);
               de4 += readlane(qv, l);
               const int32_t* col = G4 + (c * ARP_WAVE + l) * D;
@@ -192,27 +198,24 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
                 for (int cc = 0; cc < CPL; ++cc) w[cc] -= col[cc * ARP_WAVE + lane];
               }
               Sm[p][c] ^= 1ull << l;
-              ++flips;
-              last += c * ARP_WAVE + l == D - 1;
-              rem &= (~1ull) << l;
+              fc[c] += lane == l ? 1 : 0;
+              tq[c] = lane <= l ? INT32_MIN : tq[c];   // the scan resumes after lane l
+              qv = ((w[c] ^ sg[c]) - sg[c]) + dj2[c];
+              m = ballot(qv < tq[c]);
             }
           }
         } else {
           // 'par': every lane of the pass decided on one snapshot, the flips then applied in index order
           uint64_t fm[CPL];
 #pragma unroll
-          for (int c = 0; c < CPL; ++c) {
-            const int32_t qv = (((Sm[p][c] >> lane) & 1ull) ? w[c] : -w[c]) + dj2[c];
-            fm[c] = ballot(qv < tq[c]) & vmask[c];
-          }
+          for (int c = 0; c < CPL; ++c) fm[c] = ballot(((w[c] ^ sg[c]) - sg[c]) + dj2[c] < tq[c]);
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             uint64_t m = fm[c];
             while (m) {
               const int l = __builtin_ctzll(m);
               m &= m - 1;
-              const int32_t qv = (((Sm[p][c] >> lane) & 1ull) ? w[c] : -w[c]) + dj2[c];
-              de4 += readlane(qv, l);
+              de4 += readlane(((w[c] ^ sg[c]) - sg[c]) + dj2[c], l);
               const int32_t* col = G4 + (c * ARP_WAVE + l) * D;
               if ((Sm[p][c] >> l) & 1ull) {
 #pragma unroll
@@ -222,7 +225,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
                 for (int cc = 0; cc < CPL; ++cc) w[cc] -= col[cc * ARP_WAVE + lane];
               }
               Sm[p][c] ^= 1ull << l;
-              ++flips;
+              fc[c] += lane == l ? 1 : 0;
             }
           }
         }
@@ -244,20 +247,30 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
       for (int p = 0; p < P; ++p) out_s[t * (size_t)(P * D) + p * D + j] = ((Bm[p][c] >> lane) & 1ull) ? 1 : -1;
     }
   }
+  // flips = the wave's sum of the lane counts; tests = L + flips - flips on the last lane (D - 1)
+  int fl = 0;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) fl += fc[c];
+#pragma unroll
+  for (int o = 32; o; o >>= 1) fl += __shfl_xor(fl, o);
+  int last = 0;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c)
+    if (c == (D - 1) / ARP_WAVE) last = readlane(fc[c], (D - 1) % ARP_WAVE);
   if (lane == 0) {
     out_de[t] = 4 * best4;
-    out_st[2 * t] = flips;
-    out_st[2 * t + 1] = mode == 0 ? (long long)L + flips - last : (long long)L;
+    out_st[2 * t] = fl;
+    out_st[2 * t + 1] = MODE == 0 ? (long long)L + fl - last : (long long)L;
   }
 }
 
 // grid.x = B frames, block = 64 W threads. LDS: 4 Gt, D*D int32 (+ one wave of padding so that the lanes past D of
 // the last column stay inside the allocation).
-template <int P, int CPL>
+template <int P, int CPL, int MODE>
 __global__ __launch_bounds__(256) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt, const int32_t* __restrict__ h,
                                                           const int32_t* __restrict__ T, int L, int q, int R, int D,
                                                           const uint32_t* __restrict__ inst, uint32_t k0, uint32_t k1,
-                                                          int start, int mode, int8_t* __restrict__ out_s,
+                                                          int start, int8_t* __restrict__ out_s,
                                                           long long* __restrict__ out_de,
                                                           long long* __restrict__ out_st,
                                                           const uint32_t* __restrict__ thr, int gate,
@@ -271,8 +284,20 @@ __global__ __launch_bounds__(256) void arpeggio_fx_kernel(const int32_t* __restr
   const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
   const uint32_t in = inst[b];
   for (int r = wv; r < R; r += W)
-    arpeggio_anneal<P, CPL>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)r, in, k0, k1, start, mode, out_s, out_de,
-                            out_st, (size_t)b * R + r, thr ? thr + (size_t)b * P * D : nullptr, gate, used);
+    arpeggio_anneal<P, CPL, MODE>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)r, in, k0, k1, start, out_s, out_de,
+                                  out_st, (size_t)b * R + r, thr ? thr + (size_t)b * P * D : nullptr, gate, used);
+}
+
+template <int P, int CPL, int MODE>
+static void launch_pcm(dim3 grid, dim3 block, size_t lds, hipStream_t st, const int32_t* Gt, const int32_t* h,
+                       const int32_t* T, int L, int q, int R, int D, const uint32_t* inst, uint32_t k0, uint32_t k1,
+                       int start, int8_t* out_s, long long* out_de, long long* out_st, const uint32_t* thr, int gate,
+                       int8_t* used) {
+  if (lds > 65536)
+    hipFuncSetAttribute((const void*)arpeggio_fx_kernel<P, CPL, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds);
+  hipLaunchKernelGGL((arpeggio_fx_kernel<P, CPL, MODE>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1,
+                     start, out_s, out_de, out_st, thr, gate, used);
 }
 
 template <int P>
@@ -283,16 +308,72 @@ static hipError_t launch_p(const int32_t* Gt, const int32_t* h, const int32_t* T
   const int cpl = D <= ARP_WAVE ? 1 : 2;
   const size_t lds = sizeof(int32_t) * ((size_t)D * D + ARP_WAVE * cpl);
   dim3 grid(B), block(ARP_WAVE * waves);
-  if (lds > 65536) {
-    hipFuncSetAttribute(cpl == 1 ? (const void*)arpeggio_fx_kernel<P, 1> : (const void*)arpeggio_fx_kernel<P, 2>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (cpl == 1) {
+    if (mode == 0) launch_pcm<P, 1, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, thr, gate, used);
+    else launch_pcm<P, 1, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, thr, gate, used);
+  } else {
+    if (mode == 0) launch_pcm<P, 2, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, thr, gate, used);
+    else launch_pcm<P, 2, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, thr, gate, used);
   }
-  if (cpl == 1)
-    hipLaunchKernelGGL((arpeggio_fx_kernel<P, 1>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start,
-                       mode, out_s, out_de, out_st, thr, gate, used);
-  else
-    hipLaunchKernelGGL((arpeggio_fx_kernel<P, 2>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start,
-                       mode, out_s, out_de, out_st, thr, gate, used);
+  return hipGetLastError();
+}
+
+// Persistent mixed-size launch (P = 2): `wg` workgroups of `waves` waves pull tiles from a device counter in the
+// global tile order (the caller puts the largest frames first); a tile's frames are staged side by side in LDS, each
+// as 4 Gt plus one wave of padding per 64 coordinates, and the waves run the tile's nf x R anneals. Every anneal is
+// the same function with the same arguments as in arpeggio_fx_kernel, so results equal the per-class launches.
+// The counter is cleared by arpeggio_batch_reset_kernel on the same stream before each launch.
+__global__ void arpeggio_batch_reset_kernel(unsigned* counter) { *counter = 0u; }
+
+__global__ __launch_bounds__(1024) void arpeggio_batched_kernel(ArpBatchArgs a, unsigned* __restrict__ counter) {
+  extern __shared__ int32_t sG[];
+  __shared__ int s_tile;
+  const int W = blockDim.x >> 6, wv = threadIdx.x >> 6;
+  const int R = a.R;
+  while (true) {
+    if (threadIdx.x == 0) s_tile = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int t = s_tile;
+    __syncthreads();  // every thread holds t before the next tile's fetch may overwrite s_tile
+    if (t >= a.n_tiles) break;
+    int ci = 0;
+    for (int i = 1; i < a.n_classes; ++i)
+      if (t >= a.c[i].first_tile) ci = i;
+    const ArpBatchClass& C = a.c[ci];
+    const int D = C.D, nG = D * D, cpl = D <= ARP_WAVE ? 1 : 2;
+    const int stride = nG + ARP_WAVE * cpl;
+    const int f0 = (t - C.first_tile) * C.tile;
+    const int nf = C.B - f0 < C.tile ? C.B - f0 : C.tile;
+    const int32_t* g = C.Gt + (size_t)f0 * nG;
+    for (int x = threadIdx.x; x < nf * stride; x += blockDim.x) {
+      const int f = x / stride, e = x - f * stride;
+      sG[x] = e < nG ? 4 * g[(size_t)f * nG + e] : 0;
+    }
+    __syncthreads();
+    for (int u = wv; u < nf * R; u += W) {
+      const int f = u / R, r = u - f * R, b = f0 + f;
+      if (cpl == 1)
+        arpeggio_anneal<2, 1, 0>(sG + f * stride, C.h + (size_t)b * D, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b],
+                                 a.k0, a.k1, C.start, C.out_s, C.out_de, C.out_st, (size_t)b * R + r, nullptr, 0,
+                                 nullptr);
+      else
+        arpeggio_anneal<2, 2, 0>(sG + f * stride, C.h + (size_t)b * D, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b],
+                                 a.k0, a.k1, C.start, C.out_s, C.out_de, C.out_st, (size_t)b * R + r, nullptr, 0,
+                                 nullptr);
+    }
+    __syncthreads();  // the tile's LDS is free again
+  }
+}
+
+hipError_t launch_arpeggio_batched(const ArpBatchArgs& a, unsigned* counter, int wg, int waves, size_t lds,
+                                   hipStream_t st) {
+  if (lds > 65536) {
+    const hipError_t e =
+        hipFuncSetAttribute((const void*)arpeggio_batched_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(arpeggio_batch_reset_kernel, dim3(1), dim3(1), 0, st, counter);
+  hipLaunchKernelGGL(arpeggio_batched_kernel, dim3(wg), dim3(ARP_WAVE * waves), lds, st, a, counter);
   return hipGetLastError();
 }
 

@@ -177,6 +177,58 @@ extern "C" int arp_anneal_arpeggio_ws(const int32_t* Gt, const int32_t* h, const
                    "arpeggio_ws launch");
 }
 
+extern "C" int arp_anneal_arpeggio_batched(const arp_batch_class* classes, int n_classes, int R, uint32_t k0,
+                                           uint32_t k1, uint32_t* counter, int workgroups, int waves, void* stream) {
+  if (n_classes < 1 || n_classes > ARP_MAX_CLASSES)
+    return fail(ARP_E_ARG, "arpeggio_batched: n_classes = %d outside 1..%d", n_classes, ARP_MAX_CLASSES);
+  if (!classes || !counter) return fail(ARP_E_ARG, "arpeggio_batched: null pointer");
+  if (R < 1) return fail(ARP_E_ARG, "arpeggio_batched: R = %d", R);
+  if (waves < 1 || waves > 16 || workgroups < 1)
+    return fail(ARP_E_ARG, "arpeggio_batched: workgroups = %d, waves = %d (1..16)", workgroups, waves);
+  arp::ArpBatchArgs a = {};
+  a.n_classes = n_classes;
+  a.R = R;
+  a.k0 = k0;
+  a.k1 = k1;
+  size_t lds = 0;
+  int tiles = 0;
+  for (int i = 0; i < n_classes; ++i) {
+    const arp_batch_class& c = classes[i];
+    if (c.D < 1 || c.D > 128) return fail(ARP_E_RANGE, "arpeggio_batched: class %d D = %d outside 1..128", i, c.D);
+    if (c.B < 0 || c.L < 1 || c.tile < 1 || (c.start != 0 && c.start != 1))
+      return fail(ARP_E_ARG, "arpeggio_batched: class %d B = %d, L = %d, tile = %d, start = %d", i, c.B, c.L, c.tile,
+                  c.start);
+    if (c.B > 0 && (!c.Gt || !c.h || !c.inst || !c.T || !c.out_s || !c.out_de || !c.out_st))
+      return fail(ARP_E_ARG, "arpeggio_batched: class %d null pointer", i);
+    const size_t stride = (size_t)c.D * c.D + ARP_WAVE * (c.D <= ARP_WAVE ? 1 : 2);
+    const int t = c.tile < c.B ? c.tile : (c.B > 0 ? c.B : 1);
+    lds = lds > stride * t * 4 ? lds : stride * t * 4;
+    arp::ArpBatchClass& d = a.c[i];
+    d.Gt = c.Gt;
+    d.h = c.h;
+    d.inst = c.inst;
+    d.T = c.T;
+    d.out_s = c.out_s;
+    d.out_de = (long long*)c.out_de;
+    d.out_st = (long long*)c.out_st;
+    d.D = c.D;
+    d.B = c.B;
+    d.L = c.L;
+    d.q = c.q;
+    d.start = c.start;
+    d.tile = t;
+    d.first_tile = tiles;
+    tiles += (c.B + t - 1) / t;
+  }
+  a.n_tiles = tiles;
+  if (tiles == 0) return ok();
+  if (lds > device_max_lds())
+    return fail(ARP_E_RANGE, "arpeggio_batched: tiles need %zu bytes of LDS, the device allows %zu", lds,
+                device_max_lds());
+  return hip_check(arp::launch_arpeggio_batched(a, counter, workgroups, waves, lds, (hipStream_t)stream),
+                   "arpeggio_batched launch");
+}
+
 extern "C" int arp_mlp_gemm(const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, int M, int N,
                             int K, const float* bias, int act, float* Cf, float* Cpre, uint16_t* Cb, long ldc,
                             void* stream) {

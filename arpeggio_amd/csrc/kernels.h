@@ -22,6 +22,26 @@ struct FrontOut {
   int feat_ld;
 };
 
+// One class of a mixed batch for the persistent Arpeggio launch: frames of one size D with their own buffers (the
+// layouts of launch_arpeggio_fx) and schedule; `tile` frames form one unit of work, and the class's tiles are
+// numbered from first_tile in the launch's global tile order.
+struct ArpBatchClass {
+  const int32_t* Gt;     // (B, D, D)
+  const int32_t* h;      // (B, D)
+  const uint32_t* inst;  // (B,)
+  const int32_t* T;      // (L,)
+  int8_t* out_s;         // (B, R, P D)
+  long long* out_de;     // (B, R)
+  long long* out_st;     // (B, R, 2)
+  int D, B, L, q, start, tile, first_tile, pad_;
+};
+#define ARP_MAX_CLASSES 8
+struct ArpBatchArgs {
+  ArpBatchClass c[ARP_MAX_CLASSES];
+  int n_classes, n_tiles, R, pad_;
+  uint32_t k0, k1;
+};
+
 // number of learned-start features for D coordinates (before padding)
 __host__ __device__ inline int feat_count(int D) { return 4 * D + 1; }
 
@@ -29,6 +49,8 @@ hipError_t launch_arpeggio_fx(const int32_t* Gt, const int32_t* h, const int32_t
                               int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode,
                               int8_t* out_s, long long* out_de, long long* out_st, int waves, hipStream_t st,
                               const uint32_t* thr = nullptr, int gate = 0, int8_t* used = nullptr);
+hipError_t launch_arpeggio_batched(const ArpBatchArgs& a, unsigned* counter, int wg, int waves, size_t lds,
+                                   hipStream_t st);
 hipError_t launch_chord_fx(const int16_t* J, const int32_t* h, const int32_t* h0, const int32_t* Tq,
                            const int32_t* Cq, int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1,
                            int inst0, int mode, int32_t* out_e, int8_t* out_s, unsigned long long* stats, int waves,

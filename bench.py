@@ -112,14 +112,18 @@ def calibrate(args):
     return round(s0 + t * (s1 - s0), 2), pts
 
 
-def time_pipe(pipe, b, steps, warmup):
-    """K timed steps after W warm-up steps; barrier = synchronize on both sides. Returns det/s, ms/step, mean anneal
-    kernel ms (HIP events on the launch stream), BER over the timed steps."""
+def time_pipe(pipe, b, steps, warmup, min_warm_s=1.0):
+    """K timed steps after at least W warm-up steps and at least min_warm_s seconds of warm-up work (the GPU's clocks
+    settle under load); barrier = synchronize on both sides. Returns det/s, ms/step, mean anneal kernel ms (HIP
+    events on the launch stream), BER over the timed steps."""
     import torch
     lt = b['l'].to(torch.int8).contiguous()
     H, y, N0 = b['H'], b['y'], b['N0']
-    for _ in range(warmup):
+    t_w, n_w = time.perf_counter(), 0
+    while n_w < warmup or time.perf_counter() - t_w < min_warm_s:
         pipe.run(H, y, N0, lt)
+        torch.cuda.synchronize()
+        n_w += 1
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     errs = torch.zeros((), dtype=torch.int64, device='cuda')
     torch.cuda.synchronize()

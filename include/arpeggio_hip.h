@@ -113,6 +113,29 @@ int arp_anneal_arpeggio_ws(const int32_t *Gt, const int32_t *h, const int32_t *T
                            int B, const uint32_t *inst, uint32_t k0, uint32_t k1, const uint32_t *thr, int gate,
                            int8_t *out_s, int64_t *out_de, int64_t *out_st, int8_t *out_used, void *stream);
 
+/* One class of a mixed batch (frames of one size D): device buffers with the layouts of arp_anneal_arpeggio, that
+ * class's schedule T (L int32), q, start (0 rand, 1 bal) and `tile` = frames per unit of work (their staged Grams must
+ * fit the workgroup's LDS together: tile * (D*D + 64*ceil(D/64)) * 4 bytes). */
+typedef struct {
+  const int32_t *Gt;
+  const int32_t *h;
+  const uint32_t *inst;
+  const int32_t *T;
+  int8_t *out_s;
+  int64_t *out_de;
+  int64_t *out_st;
+  int D, B, L, q, start, tile;
+} arp_batch_class;
+
+/* Frames of uneven size in one persistent fixed-point Arpeggio launch (P = 2, first accept): `workgroups`
+ * workgroups of `waves` wave64s pull tiles of frames from a device counter, in the order the classes are given (put
+ * the largest first). Each anneal equals arp_anneal_arpeggio's on the same class buffers, stream for stream. The
+ * reference launches one fixed shape per call (arpeggio_fx.cu:35-37); packing several shapes into one launch has no
+ * counterpart there. counter: one device uint32 of scratch (cleared on the stream before the launch).
+ * n_classes <= 8, D <= 128 per class, waves 1..16. */
+int arp_anneal_arpeggio_batched(const arp_batch_class *classes, int n_classes, int R, uint32_t k0, uint32_t k1,
+                                uint32_t *counter, int workgroups, int waves, void *stream);
+
 /* Learned-start MLP kernels (no reference counterpart). bf16 tensors are passed as their uint16 bit patterns.
  * arp_mlp_gemm: C(m, n) = sum_k A(m, k) B(k, n) (+ bias[n]), act 0 none / 1 GELU (tanh form), on v_mfma_f32_16x16x32_bf16
  *   with fp32 accumulation; A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]; outputs (each may be NULL): Cf

@@ -38,3 +38,16 @@ def test_argument_errors_without_gpu():
     rc = L.arp_anneal_chord(None, None, None, None, None, 0, 8, 2, 2, 48, 1, 0, 0, 0, 0, None, None, None, None)
     assert rc == -3
     assert L.arp_anneal_arpeggio(None, None, None, 8, 0, 2, 2, 16, 0, None, 0, 0, 0, 0, None, None, None, None) == 0
+
+
+def test_batched_argument_errors_without_gpu():
+    import ctypes as ct
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    arr = (_lib.BatchClass * 2)()
+    arr[0] = _lib.BatchClass(None, None, None, None, None, None, None, 200, 0, 8, 0, 0, 1)
+    rc = L.arp_anneal_arpeggio_batched(ct.cast(arr, ct.c_void_p), 1, 4, 0, 0, ct.c_void_p(16), 4, 4, None)
+    assert rc == -3 and b'D = 200' in L.arp_last_error()
+    assert L.arp_anneal_arpeggio_batched(ct.cast(arr, ct.c_void_p), 0, 4, 0, 0, ct.c_void_p(16), 4, 4, None) == -1
+    arr[0] = _lib.BatchClass(None, None, None, None, None, None, None, 64, 0, 8, 0, 0, 1)
+    assert L.arp_anneal_arpeggio_batched(ct.cast(arr, ct.c_void_p), 1, 4, 0, 0, ct.c_void_p(16), 4, 4, None) == 0

@@ -1,0 +1,99 @@
+"""GPU parity of the mixed-size persistent launch: every anneal equals the per-class arp_anneal_arpeggio launch (and
+the C oracle) bit for bit, for several launch shapes and the autotuned one; the captured HIP graph of the whole step
+replays to the eager results."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), '..', 'oracle'))
+
+
+def say(*a):
+    print('[case]', *a, flush=True)
+
+
+def per_class(det, c):
+    from arpeggio_amd import _lib
+    B, D, R = c['B'], c['D'], det.R
+    s = torch.empty(B, R, 2 * D, dtype=torch.int8, device='cuda')
+    de = torch.empty(B, R, dtype=torch.int64, device='cuda')
+    st = torch.empty(B, R, 2, dtype=torch.int64, device='cuda')
+    p = _lib.ptr
+    _lib.check(_lib.load().arp_anneal_arpeggio(p(c['Gi']), p(c['hi']), p(det.T), det.L, det.q, 2, R, D, B, p(c['inst']),
+                                               det.key & 0xFFFFFFFF, det.key >> 32, 0, 0, p(s), p(de), p(st),
+                                               _lib.stream_ptr()), 'arpeggio')
+    torch.cuda.synchronize()
+    return s, de, st
+
+
+@pytest.fixture(scope='module')
+def mixed():
+    from arpeggio_amd.batch import draw_mixed, prepare
+    say('draw mixed')
+    return prepare(draw_mixed(5, 300, snr_lo=10.0, snr_hi=22.0))
+
+
+@pytest.mark.parametrize('shape', [(3, 4, 66048), (7, 16, 163840), (1, 1, 66048), (64, 8, 81920)])
+def test_batched_equals_per_class(mixed, shape):
+    from arpeggio_amd.batch import BatchedDetector
+    say('batched', shape, [(f['nt'], f['M'], f['l'].shape[0]) for f in mixed])
+    det = BatchedDetector([dict(nt=f['nt'], M=f['M'], B=f['l'].shape[0], idx=f['idx']) for f in mixed], R=6,
+                          passes=32, key=0x1234ABCD5678, shape=shape)
+    det.run(mixed)
+    torch.cuda.synchronize()
+    for c in det.cls:
+        s, de, st = per_class(det, c)
+        assert torch.equal(s, c['spins']) and torch.equal(de, c['de']) and torch.equal(st, c['st']), (c['nt'], c['M'])
+
+
+def test_batched_vs_oracle(mixed, orc):
+    from arpeggio_amd.batch import BatchedDetector
+    say('batched vs oracle')
+    det = BatchedDetector([dict(nt=f['nt'], M=f['M'], B=f['l'].shape[0], idx=f['idx']) for f in mixed], R=4,
+                          passes=16, key=77)
+    det.run(mixed)
+    torch.cuda.synchronize()
+    T = det.T.cpu().numpy()
+    for c in det.cls:
+        n = min(4, c['B'])
+        G = c['Gi'][:n].cpu().numpy()
+        h = c['hi'][:n].cpu().numpy()
+        inst = c['inst'][:n].cpu().numpy().astype(np.int64) & 0xFFFFFFFF
+        s, de, st = orc.arpeggio_fx(G, h, 2, 4, T, det.q, inst, 77, 'rand')
+        assert (c['spins'][:n].cpu().numpy() == s).all() and (c['de'][:n].cpu().numpy() == de).all()
+        assert (c['st'][:n].cpu().numpy() == st).all()
+
+
+def test_graph_replay_and_autotune(mixed):
+    from arpeggio_amd.autotune import tune_batched
+    from arpeggio_amd.batch import BatchedDetector
+    say('graph + autotune')
+    det = BatchedDetector([dict(nt=f['nt'], M=f['M'], B=f['l'].shape[0], idx=f['idx']) for f in mixed], R=8,
+                          passes=32, key=9)
+    best_ms, shape, tried = tune_batched(det, reps=1)
+    say('autotuned', shape, best_ms, len(tried))
+    assert shape in [t[1] for t in tried]
+    det.run(mixed)
+    torch.cuda.synchronize()
+    ref = [(c['spins'].clone(), c['de'].clone(), c['st'].clone(), c['lhat'].clone(), c['biterr'].clone())
+           for c in det.cls]
+    g = det.capture(mixed)
+    for k in range(3):
+        for c in det.cls:
+            for t in ('spins', 'de', 'st', 'lhat', 'biterr'):
+                c[t].zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        for c, r in zip(det.cls, ref):
+            got = (c['spins'], c['de'], c['st'], c['lhat'], c['biterr'])
+            assert all(torch.equal(a, b) for a, b in zip(got, r)), ('replay', k, c['nt'], c['M'])
+        det.run(mixed)             # eager work between replays
+        torch.cuda.synchronize()
+    errs = det.bit_errors()
+    say('bit errors', errs)
+    assert errs >= 0
