@@ -18,12 +18,13 @@ namespace arp {
 
 // One anneal per wave. G4 is the frame's Gram column block in LDS pre-scaled by 4 (G4[j*D + i] = 4 G_ij), so a flip of
 // lane j with old spin s adds s*G4[j][i] to every w_i and 4 (s w_j + 2 G_jj) to de.
-// Spins: Sm[p][c] is a wave-uniform mask (bit l: spin of copy p at coordinate c*64 + l is +1), so committing a flip
-// and saving the best state are scalar operations.
-// Test: lane j of copy p flips iff q_j = s w_j + 2 G_jj < thr(x, 4 T_k) - B_k, with q_j formed on the vector unit
-// (two ops) each time the scan re-tests; the committed lane's q_j is also its energy step (de += 4 q_j), so a flip
-// reads one lane value. The scalar unit issues one instruction per SIMD every 4 cycles, the same rate as a wave64
-// vector instruction, so the loop keeps per-flip scalar work to the mask and counter updates.
+// Spins: wave masks (bit l of Sm[p][c]: spin of copy p at coordinate c*64 + l is +1) while the start is built, then
+// per-lane signs (0 / -1) for the passes. Test: lane j of copy p flips iff q_j = s w_j + 2 G_jj <= tq_j, tq_j the
+// high word of one multiply-add per pass; q_j is two vector ops from the sign, and the committed lane's q_j is also
+// its energy step (de += 4 q_j), so a flip reads two lane values (q and the sign). A pass's flips gather in a scalar
+// mask that updates the signs, the flip count and the last-lane count once at the end of the pass. The scalar unit
+// and the vector unit each issue about one wave instruction per 4 cycles per SIMD, so the loop splits its work
+// between them (per committed flip about ten of each).
 // tests (the reference's comparator passes) = L + flips - flips on the last lane (arpeggio_ref59.py:60-71).
 // w_i = h_i - 2 sum_j G_ij m_j = h_i - sum_j G4_ij (m_j / 2) for the state in Sm (m_j = sum_p s_pj, even: P is even)
 template <int P, int CPL>
@@ -69,7 +70,7 @@ __device__ __forceinline__ long long start_energy(const int32_t* __restrict__ hb
   return e;
 }
 
-template <int P, int CPL, int MODE>
+template <int P, int CPL, int MODE, bool FULL>
 __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb,
                                                 const int32_t* __restrict__ T, int L, int q, int D, uint32_t r,
                                                 uint32_t inst, uint32_t k0, uint32_t k1, int start,
@@ -81,7 +82,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
   bool valid[CPL];
   int32_t w[CPL], dj2[CPL];
   uint64_t vmask[CPL];
-  uint64_t Sm[P][CPL], Bm[P][CPL];
+  uint64_t Sm[P][CPL], Rm[P][CPL];  // start spins as wave masks; Rm: the random start beside a learned one
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     const int j = c * ARP_WAVE + lane;
@@ -90,7 +91,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
     vmask[c] = nv >= ARP_WAVE ? ~0ull : (nv <= 0 ? 0ull : ((1ull << nv) - 1ull));
     dj2[c] = valid[c] ? G4[j * D + j] >> 1 : 0;
     if (thr) {
-      // learned start: spin +1 iff x + thr_pj >= 2^32, x the random start's word; Bm holds the random start for now
+      // learned start: spin +1 iff x + thr_pj >= 2^32, x the random start's word; Rm holds the random start
 #pragma unroll
       for (int p = 0; p < P; ++p) {
         uint32_t x = 0u, th = 0u;
@@ -99,7 +100,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
           th = thr[p * D + j];
         }
         Sm[p][c] = ballot(valid[c] && (uint64_t)x + th >= 0x100000000ull);
-        Bm[p][c] = ballot(valid[c] && x >= 0x80000000u);
+        Rm[p][c] = ballot(valid[c] && x >= 0x80000000u);
       }
     } else if (start == 0) {
 #pragma unroll
@@ -112,40 +113,43 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
 #pragma unroll
       for (int p = 0; p < P; ++p) Sm[p][c] = ballot(valid[c] && ((bits >> p) & 1u));
     }
-    if (!thr) {
-#pragma unroll
-      for (int p = 0; p < P; ++p) Bm[p][c] = Sm[p][c];
-    }
   }
   init_fields<P, CPL>(G4, hb, D, valid, Sm, w);
   if (thr) {
-    // energy gate: keep the learned start unless the random start (in Bm) has lower energy
+    // energy gate: keep the learned start unless the random start (Rm) has lower energy
     bool take = true;
     if (gate) {
       int32_t wr[CPL];
-      init_fields<P, CPL>(G4, hb, D, valid, Bm, wr);
+      init_fields<P, CPL>(G4, hb, D, valid, Rm, wr);
       const long long el = start_energy<P, CPL>(hb, valid, Sm, w);
-      const long long er = start_energy<P, CPL>(hb, valid, Bm, wr);
+      const long long er = start_energy<P, CPL>(hb, valid, Rm, wr);
       take = el <= er;
       if (!take) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           w[c] = wr[c];
 #pragma unroll
-          for (int p = 0; p < P; ++p) Sm[p][c] = Bm[p][c];
+          for (int p = 0; p < P; ++p) Sm[p][c] = Rm[p][c];
         }
       }
     }
-#pragma unroll
-    for (int c = 0; c < CPL; ++c)
-#pragma unroll
-      for (int p = 0; p < P; ++p) Bm[p][c] = Sm[p][c];
     if (used && lane == 0) used[t] = take ? 1 : 0;
   }
+  // From here the spins live as per-lane signs: sgp[p][c] = 0 / -1 for spin +1 / -1 of copy p at coordinate
+  // c*64 + lane (Bs: the best state's).
+  int32_t sgp[P][CPL], Bs[P][CPL];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      sgp[p][c] = __builtin_amdgcn_inverse_ballot_w64(Sm[p][c]) ? 0 : -1;
+      Bs[p][c] = sgp[p][c];
+    }
   long long de4 = 0, best4 = 1LL << 60;   // de / 4
   int32_t fc[CPL];                        // this lane's flip count
 #pragma unroll
   for (int c = 0; c < CPL; ++c) fc[c] = 0;
+  const PhiloxKeys K = philox_keys_v(k0, k1);
   uint32_t rw[P][CPL][4];
   for (int kb = 0; kb < L; kb += 4 * P) {
     const int n4 = kb / (4 * P);
@@ -157,12 +161,12 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
 #pragma unroll
       for (int p = 0; p < P; ++p) {
         const int k = kb + nm * P + p;
-        if (k >= L) break;
+        if (!FULL && k >= L) break;
         if (nm == 0) {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             uint32_t c0 = (uint32_t)n4, c1 = (uint32_t)(p * D + c * ARP_WAVE + lane), c2 = r, c3 = inst;
-            philox10(c0, c1, c2, c3, k0, k1);
+            philox10_kv(c0, c1, c2, c3, K);
             rw[p][c][0] = c0;
             rw[p][c][1] = c1;
             rw[p][c][2] = c2;
@@ -171,70 +175,71 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
         }
         const int32_t Tk = Tb[nm * P + p];
         const uint32_t A = 4u * (uint32_t)Tk;
+        // Decision x A > (q_j + B_k) 2^32 (B_k = q + 2 T_k, q_j = s w_j + 2 G_jj) as q_j <= tq with
+        // tq = floor((x A - 1) / 2^32) - B_k: the high word of one 32x32+64 multiply-add with addend -1 (x A = 0
+        // gives -1), then one subtraction; the documented ranges keep tq in int32.
         const int32_t Bk = q + 2 * Tk;
-        // Per pass: tq = the decision bound (lanes past D never accept), sg = 0 / -1 for spin +1 / -1 of copy p at
-        // the start of the pass. Within a pass a lane flips at most once and is then behind the scan, so the lanes
-        // still to be tested keep their start-of-pass spins and q = s w + 2 G_jj = (w ^ sg) - sg + dj2.
-        int32_t tq[CPL], sg[CPL];
+        int32_t tq[CPL];
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          tq[c] = valid[c] ? accept_threshold(rw[p][c][nm], A) - Bk : INT32_MIN;
-          sg[c] = ((Sm[p][c] >> lane) & 1ull) ? 0 : -1;
-        }
+        for (int c = 0; c < CPL; ++c) tq[c] = (int32_t)(((uint64_t)rw[p][c][nm] * A + ~0ull) >> 32) - Bk;
+        // Within a pass a lane flips at most once and is then behind the scan, so the lanes still to be tested keep
+        // their start-of-pass signs sg and q = (w ^ sg) - sg + dj2.
         if (MODE == 0) {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
-            int32_t qv = ((w[c] ^ sg[c]) - sg[c]) + dj2[c];
-            uint64_t m = ballot(qv < tq[c]);
+            // fm collects the pass's flips; the signs take them at the end of the pass. The committed lane's old
+            // spin is its start-of-pass sign, read from that lane.
+            const int32_t sg = sgp[p][c];
+            uint64_t rem = vmask[c], fm = 0;  // rem: lanes the scan has not passed yet
+            int32_t qv = ((w[c] ^ sg) - sg) + dj2[c];
+            uint64_t m = ballot(qv <= tq[c]) & rem;
             while (m) {
               const int l = __builtin_ctzll(m);
               de4 += readlane(qv, l);
+              const int32_t sl = readlane(sg, l);   // 0: old spin +1 (w += 4 G col), -1: old spin -1 (w -= ...)
               const int32_t* col = G4 + (c * ARP_WAVE + l) * D;
-              if ((Sm[p][c] >> l) & 1ull) {
 #pragma unroll
-                for (int cc = 0; cc < CPL; ++cc) w[cc] += col[cc * ARP_WAVE + lane];
-              } else {
-#pragma unroll
-                for (int cc = 0; cc < CPL; ++cc) w[cc] -= col[cc * ARP_WAVE + lane];
-              }
-              Sm[p][c] ^= 1ull << l;
-              fc[c] += lane == l ? 1 : 0;
-              tq[c] = lane <= l ? INT32_MIN : tq[c];   // the scan resumes after lane l
-              qv = ((w[c] ^ sg[c]) - sg[c]) + dj2[c];
-              m = ballot(qv < tq[c]);
+              for (int cc = 0; cc < CPL; ++cc) w[cc] += (col[cc * ARP_WAVE + lane] ^ sl) - sl;
+              mask_set(fm, l);
+              rem &= (~1ull) << l;
+              qv = ((w[c] ^ sg) - sg) + dj2[c];
+              m = ballot(qv <= tq[c]) & rem;
             }
+            const bool fl = __builtin_amdgcn_inverse_ballot_w64(fm);
+            sgp[p][c] ^= fl ? -1 : 0;
+            fc[c] += fl ? 1 : 0;
           }
         } else {
           // 'par': every lane of the pass decided on one snapshot, the flips then applied in index order
           uint64_t fm[CPL];
 #pragma unroll
-          for (int c = 0; c < CPL; ++c) fm[c] = ballot(((w[c] ^ sg[c]) - sg[c]) + dj2[c] < tq[c]);
+          for (int c = 0; c < CPL; ++c)
+            fm[c] = ballot(((w[c] ^ sgp[p][c]) - sgp[p][c]) + dj2[c] <= tq[c]) & vmask[c];
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
+            const int32_t sg = sgp[p][c];
             uint64_t m = fm[c];
             while (m) {
               const int l = __builtin_ctzll(m);
               m &= m - 1;
-              de4 += readlane(((w[c] ^ sg[c]) - sg[c]) + dj2[c], l);
+              de4 += readlane(((w[c] ^ sg) - sg) + dj2[c], l);
+              const int32_t sl = readlane(sg, l);
               const int32_t* col = G4 + (c * ARP_WAVE + l) * D;
-              if ((Sm[p][c] >> l) & 1ull) {
 #pragma unroll
-                for (int cc = 0; cc < CPL; ++cc) w[cc] += col[cc * ARP_WAVE + lane];
-              } else {
-#pragma unroll
-                for (int cc = 0; cc < CPL; ++cc) w[cc] -= col[cc * ARP_WAVE + lane];
-              }
-              Sm[p][c] ^= 1ull << l;
-              fc[c] += lane == l ? 1 : 0;
+              for (int cc = 0; cc < CPL; ++cc) w[cc] += (col[cc * ARP_WAVE + lane] ^ sl) - sl;
             }
+            const bool fl = __builtin_amdgcn_inverse_ballot_w64(fm[c]);
+            sgp[p][c] ^= fl ? -1 : 0;
+            fc[c] += fl ? 1 : 0;
           }
         }
-        if (de4 - best4 < 0) {
+        // best-of-trajectory, strict (the first minimum wins)
+        if (de4 < best4) {
           best4 = de4;
 #pragma unroll
           for (int pp = 0; pp < P; ++pp)
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) Bm[pp][c] = Sm[pp][c];
+            for (int c = 0; c < CPL; ++c) Bs[pp][c] = sgp[pp][c];
         }
       }
     }
@@ -244,7 +249,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
     const int j = c * ARP_WAVE + lane;
     if (valid[c]) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) out_s[t * (size_t)(P * D) + p * D + j] = ((Bm[p][c] >> lane) & 1ull) ? 1 : -1;
+      for (int p = 0; p < P; ++p) out_s[t * (size_t)(P * D) + p * D + j] = Bs[p][c] ? -1 : 1;
     }
   }
   // flips = the wave's sum of the lane counts; tests = L + flips - flips on the last lane (D - 1)
@@ -266,7 +271,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
 
 // grid.x = B frames, block = 64 W threads. LDS: 4 Gt, D*D int32 (+ one wave of padding so that the lanes past D of
 // the last column stay inside the allocation).
-template <int P, int CPL, int MODE>
+template <int P, int CPL, int MODE, bool FULL>
 __global__ __launch_bounds__(256) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt, const int32_t* __restrict__ h,
                                                           const int32_t* __restrict__ T, int L, int q, int R, int D,
                                                           const uint32_t* __restrict__ inst, uint32_t k0, uint32_t k1,
@@ -284,20 +289,34 @@ __global__ __launch_bounds__(256) void arpeggio_fx_kernel(const int32_t* __restr
   const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
   const uint32_t in = inst[b];
   for (int r = wv; r < R; r += W)
-    arpeggio_anneal<P, CPL, MODE>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)r, in, k0, k1, start, out_s, out_de,
+    arpeggio_anneal<P, CPL, MODE, FULL>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)r, in, k0, k1, start, out_s, out_de,
                                   out_st, (size_t)b * R + r, thr ? thr + (size_t)b * P * D : nullptr, gate, used);
 }
 
+template <int P, int CPL, int MODE, bool FULL>
+static void launch_pcmf(dim3 grid, dim3 block, size_t lds, hipStream_t st, const int32_t* Gt, const int32_t* h,
+                        const int32_t* T, int L, int q, int R, int D, const uint32_t* inst, uint32_t k0, uint32_t k1,
+                        int start, int8_t* out_s, long long* out_de, long long* out_st, const uint32_t* thr, int gate,
+                        int8_t* used) {
+  if (lds > 65536)
+    hipFuncSetAttribute((const void*)arpeggio_fx_kernel<P, CPL, MODE, FULL>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((arpeggio_fx_kernel<P, CPL, MODE, FULL>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0,
+                     k1, start, out_s, out_de, out_st, thr, gate, used);
+}
+
+// FULL: L is a whole number of 4 P-pass Philox blocks, so the pass loop needs no bound test
 template <int P, int CPL, int MODE>
 static void launch_pcm(dim3 grid, dim3 block, size_t lds, hipStream_t st, const int32_t* Gt, const int32_t* h,
                        const int32_t* T, int L, int q, int R, int D, const uint32_t* inst, uint32_t k0, uint32_t k1,
                        int start, int8_t* out_s, long long* out_de, long long* out_st, const uint32_t* thr, int gate,
                        int8_t* used) {
-  if (lds > 65536)
-    hipFuncSetAttribute((const void*)arpeggio_fx_kernel<P, CPL, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds);
-  hipLaunchKernelGGL((arpeggio_fx_kernel<P, CPL, MODE>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1,
-                     start, out_s, out_de, out_st, thr, gate, used);
+  if (L % (4 * P) == 0)
+    launch_pcmf<P, CPL, MODE, true>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de,
+                                    out_st, thr, gate, used);
+  else
+    launch_pcmf<P, CPL, MODE, false>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de,
+                                     out_st, thr, gate, used);
 }
 
 template <int P>
@@ -350,16 +369,27 @@ __global__ __launch_bounds__(1024) void arpeggio_batched_kernel(ArpBatchArgs a, 
       sG[x] = e < nG ? 4 * g[(size_t)f * nG + e] : 0;
     }
     __syncthreads();
+    const bool full = C.L % 8 == 0;
     for (int u = wv; u < nf * R; u += W) {
       const int f = u / R, r = u - f * R, b = f0 + f;
-      if (cpl == 1)
-        arpeggio_anneal<2, 1, 0>(sG + f * stride, C.h + (size_t)b * D, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b],
-                                 a.k0, a.k1, C.start, C.out_s, C.out_de, C.out_st, (size_t)b * R + r, nullptr, 0,
-                                 nullptr);
-      else
-        arpeggio_anneal<2, 2, 0>(sG + f * stride, C.h + (size_t)b * D, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b],
-                                 a.k0, a.k1, C.start, C.out_s, C.out_de, C.out_st, (size_t)b * R + r, nullptr, 0,
-                                 nullptr);
+      const int32_t* gs = sG + f * stride;
+      const int32_t* hb = C.h + (size_t)b * D;
+      const size_t at = (size_t)b * R + r;
+      if (cpl == 1) {
+        if (full)
+          arpeggio_anneal<2, 1, 0, true>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
+                                         C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+        else
+          arpeggio_anneal<2, 1, 0, false>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
+                                          C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+      } else {
+        if (full)
+          arpeggio_anneal<2, 2, 0, true>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
+                                         C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+        else
+          arpeggio_anneal<2, 2, 0, false>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
+                                          C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+      }
     }
     __syncthreads();  // the tile's LDS is free again
   }

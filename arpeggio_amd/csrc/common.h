@@ -30,6 +30,41 @@ __device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c
   }
 }
 
+// Round keys held in vector registers (every lane the same value) for use inside long loops: the scalar register
+// file stays free for the loop's masks and counters, and the key XORs read a vector operand at the same cost.
+struct PhiloxKeys {
+  uint32_t k0[10], k1[10];
+};
+
+__device__ __forceinline__ PhiloxKeys philox_keys_v(uint32_t k0, uint32_t k1) {
+  PhiloxKeys K;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    uint32_t a = k0 + (uint32_t)i * 0x9E3779B9u, b = k1 + (uint32_t)i * 0xBB67AE85u;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(K.k0[i]) : "s"(a));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(K.k1[i]) : "s"(b));
+  }
+  return K;
+}
+
+__device__ __forceinline__ void philox10_kv(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+                                            const PhiloxKeys& K) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t x0 = hi1 ^ c1 ^ K.k0[i], x2 = hi0 ^ c3 ^ K.k1[i];
+    c0 = x0;
+    c1 = lo1;
+    c2 = x2;
+    c3 = lo0;
+  }
+}
+
+// sets bit l of a wave-uniform 64-bit mask with one scalar instruction
+__device__ __forceinline__ void mask_set(uint64_t& m, int l) { asm("s_bitset1_b64 %0, %1" : "+s"(m) : "s"(l) : "scc"); }
+
 __device__ __forceinline__ uint32_t philox_w0(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                                               uint32_t k1) {
   philox10(c0, c1, c2, c3, k0, k1);

@@ -1,0 +1,102 @@
+#!/usr/bin/env python3
+"""Profiling driver for the detector's kernels, meant to run under rocprofv3.
+
+    python3 -m arpeggio_amd.profile_anneal run [--start rand|learned] [--passes 64] [--frames 65536] [--copies 16]
+                                               [--launches 3] [--snr 23.68]
+        draws the 32x32 16-QAM frames once, then runs the full step (front end [+ MLP] + anneal + decode) `launches`
+        times; every kernel of the path appears `launches` times in a trace.
+    python3 -m arpeggio_amd.profile_anneal copy
+        three device copies of 256 MiB: the known-size reference for scaling FETCH_SIZE / WRITE_SIZE.
+    python3 -m arpeggio_amd.profile_anneal traffic FETCH_DB WRITE_DB COPY_FETCH_DB COPY_WRITE_DB [--match arpeggio]
+        HBM bytes per dispatch of the matching kernel from two counter runs (FETCH_SIZE, WRITE_SIZE, KiB per
+        dispatch), each scaled by what the same counter reads for the copy (bytes copied / counter bytes); prints
+        one JSON object."""
+import argparse
+import json
+import sqlite3
+import sys
+
+COPY_BYTES = 256 << 20
+
+
+def run(a):
+    import torch
+    from . import frames
+    from .detector import DetectPipeline
+    model = None
+    if a.start == 'learned':
+        from .learned import DEFAULT_WEIGHTS, WarmStartMLP
+        model = WarmStartMLP.load(DEFAULT_WEIGHTS)
+    b = frames.draw(1, a.snr, 0, a.frames, 32, 16, dev='cuda')
+    lt = b['l'].to(torch.int8).contiguous()
+    key = (0x9E3779B97F4A7C15 + 0xA513) & 0xFFFFFFFFFFFFFFFF
+    pipe = DetectPipeline(a.frames, 64, 64, 16, a.copies, cycles=a.passes, start=a.start, key=key, model=model)
+    for i in range(a.launches):
+        pipe.run(b['H'], b['y'], b['N0'], lt)
+        torch.cuda.synchronize()
+        print('# step %d done' % i, file=sys.stderr, flush=True)
+    print(json.dumps(dict(frames=a.frames, passes=a.passes, copies=a.copies, start=a.start,
+                          bit_errors=int(pipe.biterr.sum()))))
+
+
+def copy():
+    import torch
+    x = torch.empty(COPY_BYTES, dtype=torch.uint8, device='cuda')
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(copy_bytes=COPY_BYTES, launches=3)))
+
+
+def counter_means(db, counter, match):
+    c = sqlite3.connect(db)
+    vals = {}
+    for name, cn, v, disp in c.execute('select kernel_name, counter_name, value, dispatch_id from counters_collection'):
+        if cn == counter and match in name:
+            vals.setdefault(disp, 0.0)
+            vals[disp] += v
+    if not vals:
+        raise SystemExit('%s: no %s for kernels matching %r' % (db, counter, match))
+    v = sorted(vals.values())
+    return sum(v) / len(v), len(v)
+
+
+def traffic(a):
+    f_k, nf = counter_means(a.dbs[0], 'FETCH_SIZE', a.match)
+    w_k, nw = counter_means(a.dbs[1], 'WRITE_SIZE', a.match)
+    cf, _ = counter_means(a.dbs[2], 'FETCH_SIZE', a.copy_match)
+    cw, _ = counter_means(a.dbs[3], 'WRITE_SIZE', a.copy_match)
+    sf, sw = COPY_BYTES / (cf * 1024.0), COPY_BYTES / (cw * 1024.0)
+    out = dict(kernel_match=a.match, dispatches=[nf, nw], fetch_kib=f_k, write_kib=w_k,
+               copy_fetch_kib=cf, copy_write_kib=cw, fetch_scale=sf, write_scale=sw,
+               hbm_bytes_per_launch=f_k * 1024.0 * sf + w_k * 1024.0 * sw)
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest='cmd', required=True)
+    r = sub.add_parser('run')
+    r.add_argument('--start', default='rand', choices=('rand', 'learned'))
+    r.add_argument('--passes', type=int, default=64)
+    r.add_argument('--frames', type=int, default=65536)
+    r.add_argument('--copies', type=int, default=16)
+    r.add_argument('--launches', type=int, default=3)
+    r.add_argument('--snr', type=float, default=23.68)
+    sub.add_parser('copy')
+    t = sub.add_parser('traffic')
+    t.add_argument('dbs', nargs=4)
+    t.add_argument('--match', default='arpeggio_fx_kernel')
+    t.add_argument('--copy-match', default='copy')
+    a = ap.parse_args()
+    if a.cmd == 'run':
+        run(a)
+    elif a.cmd == 'copy':
+        copy()
+    else:
+        traffic(a)
+
+
+if __name__ == '__main__':
+    main()

@@ -52,7 +52,8 @@ def parse():
     ap.add_argument('--seed', type=int, default=1)
     ap.add_argument('--plain-only', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--cpu-frames', type=int, default=4096)
+    ap.add_argument('--cpu-frames', type=int, default=65536)
+    ap.add_argument('--warm-seconds', type=float, default=1.0, help='minimum warm-up time before each timed loop')
     ap.add_argument('--with-chord', action='store_true')
     ap.add_argument('--copy-probe', action='store_true', help='run only a plain device copy of known size (PMC scale)')
     return ap.parse_args()
@@ -125,16 +126,19 @@ def time_pipe(pipe, b, steps, warmup, min_warm_s=1.0):
         torch.cuda.synchronize()
         n_w += 1
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     errs = torch.zeros((), dtype=torch.int64, device='cuda')
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    g0.record()
     for i in range(steps):
         _, be = pipe.run(H, y, N0, lt, events=ev[i])
         errs += be.sum()
+    g1.record()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     B = pipe.B
-    return dict(value=B * steps / dt, ms_per_step=dt / steps * 1e3,
+    return dict(value=B * steps / dt, ms_per_step=dt / steps * 1e3, gpu_ms_per_step=g0.elapsed_time(g1) / steps,
                 anneal_ms=float(np.mean([a.elapsed_time(c) for a, c in ev])),
                 ber=float(errs.item()) / (B * 4 * NT * steps))
 
@@ -237,16 +241,16 @@ def main():
                 chosen = Ll
                 break
     log('timing plain (%d passes)' % L)
-    tp = time_pipe(plain, b, args.steps, args.warmup)
-    plain_rec = dict(value=tp['value'], ms_per_step=tp['ms_per_step'], passes=L, start='rand',
-                     ber=float(be_plain.sum()) / nbits, anneal_ms=tp['anneal_ms'])
+    tp = time_pipe(plain, b, args.steps, args.warmup, args.warm_seconds)
+    plain_rec = dict(value=tp['value'], ms_per_step=tp['ms_per_step'], gpu_ms_per_step=tp['gpu_ms_per_step'],
+                     passes=L, start='rand', ber=float(be_plain.sum()) / nbits, anneal_ms=tp['anneal_ms'])
     if chosen is not None:
         log('timing learned start (%d passes)' % chosen)
         lp = make_pipe(B, args, chosen, 'learned', model)
-        tl = time_pipe(lp, b, args.steps, args.warmup)
+        tl = time_pipe(lp, b, args.steps, args.warmup, args.warm_seconds)
         out.update(value=tl['value'], ms_per_step=tl['ms_per_step'])
-        out['learned'] = dict(passes=chosen, ms_per_step=tl['ms_per_step'], ber=tl['ber'], anneal_ms=tl['anneal_ms'],
-                              ber_matched=True)
+        out['learned'] = dict(passes=chosen, ms_per_step=tl['ms_per_step'], gpu_ms_per_step=tl['gpu_ms_per_step'],
+                              ber=tl['ber'], anneal_ms=tl['anneal_ms'], ber_matched=True)
         out['speedup_vs_plain'] = tl['value'] / tp['value']
         out['roofline'] = roofline(lp, tl, B, chosen, R, True)
         timed = 'learned'
