@@ -79,10 +79,11 @@ class DetectPipeline:
     policy 'chord': snapshot anneal on the expanded N = 2D problem, cycles = S steps, schedule (t0, t1, q, kappa)."""
 
     def __init__(self, B, Mr, D, M, R, policy='arpeggio', cycles=64, T0=0.1, T1=0.1, q=0.0, kappa=0.0,
-                 start='rand', key=0, inst0=0, mode='arpeggio', Q=QSCALE, model=None, gate=True, inv_temp=1.0):
+                 start='rand', key=0, inst0=0, mode='arpeggio', Q=QSCALE, model=None, gate=True, inv_temp=None):
         """start 'learned' (Arpeggio only) needs model (a learned.WarmStartMLP): the front end also writes the
         features, the MLP turns them into per-spin start thresholds, and the anneal starts from them (gate: fall back
-        to the random start per anneal when its energy is lower)."""
+        to the random start per anneal when its energy is lower). inv_temp scales the logits (None: the model's
+        calibrated value)."""
         self.B, self.Mr, self.D, self.M, self.R = B, Mr, D, M, R
         self.policy, self.K, self.P = policy, 1, 2
         self.key, self.inst0, self.start, self.mode = key, inst0, start, mode
@@ -107,7 +108,8 @@ class DetectPipeline:
             if start == 'learned':
                 from .learned import feat_ld
                 assert model is not None and model.D == D and model.P == self.P, 'learned start needs a matching model'
-                self.model, self.gate, self.inv_temp = model, gate, inv_temp
+                self.model, self.gate = model, gate
+                self.inv_temp = model.inv_temp if inv_temp is None else inv_temp
                 self.feat = torch.empty(B, feat_ld(D), dtype=torch.bfloat16, device=dev)
                 self.thr = torch.empty(B, N, dtype=torch.int32, device=dev)
                 self.used = torch.empty(B, R, dtype=torch.int8, device=dev)

@@ -53,6 +53,7 @@ class WarmStartMLP:
         self.moments = {k: (torch.zeros_like(v), torch.zeros_like(v)) for k, v in self.params.items()}
         self.wb = {k: _bf16(self.params[k]) for k in ('W1', 'W2', 'W3')}
         self.step_count = 0
+        self.inv_temp = 1.0     # logit scale of the start thresholds, calibrated after training
         self.lib = _lib.load()
 
     # ---------------------------------------------------------------- persistence
@@ -60,7 +61,7 @@ class WarmStartMLP:
         return {k: v.detach().cpu().numpy() for k, v in self.params.items()}
 
     def save(self, path):
-        np.savez_compressed(path, D=self.D, P=self.P, H=self.H, **self.state_dict())
+        np.savez_compressed(path, D=self.D, P=self.P, H=self.H, inv_temp=self.inv_temp, **self.state_dict())
 
     @classmethod
     def load(cls, path, device='cuda'):
@@ -69,6 +70,8 @@ class WarmStartMLP:
         for k in m.params:
             m.params[k] = torch.as_tensor(z[k]).to(device).contiguous()
         m.wb = {k: _bf16(m.params[k]) for k in ('W1', 'W2', 'W3')}
+        if 'inv_temp' in z.files:
+            m.inv_temp = float(z['inv_temp'])
         return m
 
     # ---------------------------------------------------------------- kernels

@@ -4,9 +4,13 @@
     python -m arpeggio_amd.train_warmstart [--frames 1048576] [--out arpeggio_amd/weights/....npz]
 
 Training frames: 32 users x 32 rx, 16-QAM, i.i.d. Rayleigh, per-frame SNR uniform in [snr-lo, snr-hi] dB, drawn on the
-GPU. Labels: the detection of a 1024-pass, 8-anneal first-accept run (learned.label_batch). Evaluation: held-out
-frames from the reference's host draw recipe (seed < 1000) at one SNR; bit error rates of the plain start and the
-learned start at several pass budgets, R anneals per frame. Prints one JSON line."""
+GPU. Labels: the detection of a 1024-pass, 8-anneal first-accept run (learned.label_batch). Calibration: the logit
+scale of the start thresholds (inv_temp) that gives the lowest BER at --cal-passes on held-out frames of the
+reference's host draw recipe (seed 7, draw index 3), stored with the weights. Evaluation: other held-out frames
+(seed 7, draw index 0) at one SNR; bit error rates of the plain start and the learned start at several pass budgets,
+R anneals per frame. Prints one JSON line.
+
+    python -m arpeggio_amd.train_warmstart --calibrate-only [--weights IN] [--out OUT]   (calibrate existing weights)"""
 import argparse
 import json
 import os
@@ -18,6 +22,26 @@ import torch
 from . import frames
 from .detector import DetectPipeline
 from .learned import DEFAULT_WEIGHTS, WarmStartMLP, label_batch, train
+
+
+def calibrate(model, a, log, grid=(0.25, 0.35, 0.5, 0.7, 1.0)):
+    """Sets model.inv_temp to the grid value with the lowest BER of the learned start at a.cal_passes passes on
+    held-out frames (seed 7, draw index 3, a.cal_snr dB); returns the table, with the random start beside it."""
+    B = a.cal_frames
+    b = frames.draw(7, a.cal_snr, 3, B, 32, 16, dev='cuda')
+    lt = b['l'].to(torch.int8)
+    res = {}
+    for it in (None,) + tuple(grid):
+        pipe = DetectPipeline(B, 64, 64, 16, a.R, policy='arpeggio', cycles=a.cal_passes, T0=0.1, T1=0.1, q=0.0,
+                              start='rand' if it is None else 'learned', key=1234,
+                              model=None if it is None else model, inv_temp=it)
+        _, be = pipe.run(b['H'], b['y'], b['N0'], lt)
+        ber = float(be.sum()) / (B * 128)
+        res['rand' if it is None else 'inv_temp_%g' % it] = ber
+        log('calibration', 'rand' if it is None else 'inv_temp %g' % it, '%.3e' % ber)
+    best = min(grid, key=lambda it: res['inv_temp_%g' % it])
+    model.inv_temp = float(best)
+    return res
 
 
 def main():
@@ -36,8 +60,19 @@ def main():
     ap.add_argument('--eval-snr', type=float, default=24.0)
     ap.add_argument('--eval-frames', type=int, default=65536)
     ap.add_argument('--R', type=int, default=16)
+    ap.add_argument('--cal-snr', type=float, default=23.7)
+    ap.add_argument('--cal-passes', type=int, default=64)
+    ap.add_argument('--cal-frames', type=int, default=262144)
+    ap.add_argument('--weights', default=None, help='weights to calibrate (default: --out)')
+    ap.add_argument('--calibrate-only', action='store_true')
     a = ap.parse_args()
     log = lambda *x: print('#', *x, file=sys.stderr, flush=True)
+    if a.calibrate_only:
+        model = WarmStartMLP.load(a.weights or a.out)
+        cal = calibrate(model, a, log)
+        model.save(a.out)
+        print(json.dumps(dict(calibration=cal, inv_temp=model.inv_temp, out=a.out)), flush=True)
+        return
     t0 = time.time()
     feats, labels = [], []
     for c in range(0, a.frames, a.chunk):
@@ -59,6 +94,7 @@ def main():
     losses = train(model, feats, labels, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
     torch.cuda.synchronize()
     t_train = time.time() - t1
+    cal = calibrate(model, a, log)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     model.save(a.out)
     log('trained in %.1f s, saved %s' % (t_train, a.out))
@@ -79,6 +115,7 @@ def main():
             res['%s_L%d' % (start, L)] = r
             log(start, L, r)
     print(json.dumps(dict(frames=a.frames, label_passes=a.passes, label_R=a.label_R, epochs=a.epochs,
+                          inv_temp=model.inv_temp, calibration=cal,
                           loss_first=losses[0], loss_last=losses[-1], label_s=t_label, train_s=t_train,
                           eval_snr=a.eval_snr, eval_frames=a.eval_frames, R=a.R, results=res)), flush=True)
 
