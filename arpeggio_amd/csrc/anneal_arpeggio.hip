@@ -35,16 +35,37 @@ __device__ __forceinline__ void init_fields(const int32_t* __restrict__ G4, cons
   for (int c = 0; c < CPL; ++c) w[c] = valid[c] ? hb[c * ARP_WAVE + lane] : 0;
 #pragma unroll
   for (int c2 = 0; c2 < CPL; ++c2) {
-    const int nj = D - c2 * ARP_WAVE < ARP_WAVE ? D - c2 * ARP_WAVE : ARP_WAVE;
-    for (int l = 0; l < nj; ++l) {
-      int half_m = 0;
-#pragma unroll
-      for (int p = 0; p < P; ++p) half_m += (int)((Sm[p][c2] >> l) & 1ull);
-      half_m -= P / 2;
-      if (half_m != 0) {
+    if (P == 2) {
+      // m_j / 2 = +1 where both copies are +1, -1 where both are -1, 0 otherwise: visit only those columns
+      const int nv = D - c2 * ARP_WAVE;
+      const uint64_t vm = nv >= ARP_WAVE ? ~0ull : ((1ull << nv) - 1ull);
+      uint64_t plus = Sm[0][c2] & Sm[P - 1][c2], minus = ~(Sm[0][c2] | Sm[P - 1][c2]) & vm;
+      while (plus) {
+        const int l = __builtin_ctzll(plus);
+        plus &= plus - 1;
         const int32_t* col = G4 + (c2 * ARP_WAVE + l) * D;
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) w[c] -= half_m * col[c * ARP_WAVE + lane];
+        for (int c = 0; c < CPL; ++c) w[c] -= col[c * ARP_WAVE + lane];
+      }
+      while (minus) {
+        const int l = __builtin_ctzll(minus);
+        minus &= minus - 1;
+        const int32_t* col = G4 + (c2 * ARP_WAVE + l) * D;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) w[c] += col[c * ARP_WAVE + lane];
+      }
+    } else {
+      const int nj = D - c2 * ARP_WAVE < ARP_WAVE ? D - c2 * ARP_WAVE : ARP_WAVE;
+      for (int l = 0; l < nj; ++l) {
+        int half_m = 0;
+#pragma unroll
+        for (int p = 0; p < P; ++p) half_m += (int)((Sm[p][c2] >> l) & 1ull);
+        half_m -= P / 2;
+        if (half_m != 0) {
+          const int32_t* col = G4 + (c2 * ARP_WAVE + l) * D;
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) w[c] -= half_m * col[c * ARP_WAVE + lane];
+        }
       }
     }
   }
