@@ -56,6 +56,8 @@ def parse():
     ap.add_argument('--cpu-frames', type=int, default=65536)
     ap.add_argument('--warm-seconds', type=float, default=1.0, help='minimum warm-up time before each timed loop')
     ap.add_argument('--with-chord', action='store_true')
+    ap.add_argument('--no-batched', action='store_true', help='skip the mixed-size batch measurement')
+    ap.add_argument('--batched-frames', type=int, default=1 << 17)
     ap.add_argument('--copy-probe', action='store_true', help='run only a plain device copy of known size (PMC scale)')
     return ap.parse_args()
 
@@ -206,6 +208,43 @@ def roofline(pipe, t, B, L, R, learned):
                      'decision, the ballot scan), not by HBM or the matrix cores; bytes are the algorithmic ones')
 
 
+def batched(args):
+    """Mixed-size batch: 2^17 frames, 40 / 40 / 20 % of 16 / 32 / 64 users, QPSK and 16-QAM half each, i.i.d.
+    Rayleigh, per-frame SNR uniform in 8-24 dB (GPU generator), Arpeggio R = 8, 64 passes for every class. One step =
+    per-class front end + one persistent anneal launch (autotuned shape) + per-class decode, replayed from one HIP
+    graph; K timed replays after warm-up."""
+    import torch
+    from arpeggio_amd.autotune import tune_batched
+    from arpeggio_amd.batch import BatchedDetector, draw_mixed, nbits, prepare
+    fr = prepare(draw_mixed(args.seed, args.batched_frames))
+    det = BatchedDetector([dict(nt=f['nt'], M=f['M'], B=f['l'].shape[0], idx=f['idx']) for f in fr], R=8, passes=64,
+                          key=key_of(args.seed))
+    det.run(fr)
+    torch.cuda.synchronize()
+    best_ms, shape, tried = tune_batched(det, reps=2, log=log)
+    g = det.capture(fr)
+    t_w = time.perf_counter()
+    while time.perf_counter() - t_w < args.warm_seconds:
+        g.replay()
+        torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    B = args.batched_frames
+    return dict(value=B * args.steps / dt, unit='detections/s', ms_per_step=dt / args.steps * 1e3,
+                gpu_ms_per_step=e0.elapsed_time(e1) / args.steps, frames=B,
+                classes=[dict(users=c['nt'], M=c['M'], frames=c['B'], tile=c['tile']) for c in det.cls],
+                shape=dict(workgroups=shape[0], waves=shape[1], lds_budget=shape[2]), anneal_ms=best_ms,
+                autotune=[dict(ms=ms, workgroups=sh[0], waves=sh[1], lds_budget=sh[2]) for ms, sh in tried],
+                ber=det.bit_errors() / nbits(fr), R=8, passes=64, snr_db=[8.0, 24.0], graph=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -276,6 +315,9 @@ def main():
     if not args.no_cpu:
         log('cpu baseline on %d frames' % args.cpu_frames)
         out['cpu_baseline'] = cpu_baseline(b, plain, min(args.cpu_frames, B), args)
+    if not args.no_batched:
+        log('mixed-size batch (%d frames)' % args.batched_frames)
+        out['batched'] = batched(args)
     if args.with_chord:
         from arpeggio_amd.detector import DetectPipeline
         lt = b['l'].to(torch.int8).contiguous()
