@@ -293,7 +293,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
 // grid.x = B frames, block = 64 W threads. LDS: 4 Gt, D*D int32 (+ one wave of padding so that the lanes past D of
 // the last column stay inside the allocation).
 template <int P, int CPL, int MODE, bool FULL>
-__global__ __launch_bounds__(256) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt, const int32_t* __restrict__ h,
+__global__ __launch_bounds__(512) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt, const int32_t* __restrict__ h,
                                                           const int32_t* __restrict__ T, int L, int q, int R, int D,
                                                           const uint32_t* __restrict__ inst, uint32_t k0, uint32_t k1,
                                                           int start, int8_t* __restrict__ out_s,

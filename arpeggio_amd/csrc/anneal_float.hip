@@ -16,7 +16,7 @@ __device__ __forceinline__ float uniform_r(uint32_t x, float T4) {
 }
 
 template <int P, int CPL>
-__global__ __launch_bounds__(256) void arpeggio_f32_kernel(const float* __restrict__ Gt, const float* __restrict__ h,
+__global__ __launch_bounds__(512) void arpeggio_f32_kernel(const float* __restrict__ Gt, const float* __restrict__ h,
                                                            const float* __restrict__ T, int L, float q, int R, int D,
                                                            const uint32_t* __restrict__ inst, uint32_t k0, uint32_t k1,
                                                            int start, int8_t* __restrict__ out_s,

@@ -50,6 +50,14 @@ static size_t device_max_lds() {
   return (size_t)v;
 }
 
+// Anneals (waves) per workgroup of the Arpeggio kernels: a frame of D <= 64 stages 17 KiB of Gram, so four waves
+// per workgroup already fill a CU; a frame of D <= 128 stages 65 KiB (two workgroups per CU), so eight waves. The
+// kernels are compiled for workgroups of up to 512 threads (eight waves).
+static int arpeggio_waves(int R, int D) {
+  const int w = D > 64 ? 8 : 4;
+  return R < w ? R : w;
+}
+
 extern "C" const char* arp_last_error(void) { return g_err.c_str(); }
 
 extern "C" int arp_abi_version(void) { return 1; }
@@ -74,7 +82,7 @@ extern "C" int arp_anneal_arpeggio(const int32_t* Gt, const int32_t* h, const in
   if (mode != 0 && mode != 1) return fail(ARP_E_ARG, "arpeggio: mode = %d, expected 0 or 1", mode);
   if (B == 0) return ok();
   if (!Gt || !h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio: null pointer");
-  const int waves = R < 4 ? R : 4;
+  const int waves = arpeggio_waves(R, D);
   return hip_check(arp::launch_arpeggio_fx(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, start, mode, out_s,
                                            (long long*)out_de, (long long*)out_st, waves, (hipStream_t)stream),
                    "arpeggio launch");
@@ -140,7 +148,7 @@ extern "C" int arp_anneal_arpeggio_f32(const float* Gt, const float* h, const fl
   if (start != 0 && start != 1) return fail(ARP_E_ARG, "arpeggio_f32: start = %d", start);
   if (B == 0) return ok();
   if (!Gt || !h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio_f32: null pointer");
-  const int waves = R < 4 ? R : 4;
+  const int waves = arpeggio_waves(R, D);
   return hip_check(arp::launch_arpeggio_f32(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, start, out_s, out_de,
                                             (long long*)out_st, out_w, waves, (hipStream_t)stream),
                    "arpeggio_f32 launch");
@@ -170,7 +178,7 @@ extern "C" int arp_anneal_arpeggio_ws(const int32_t* Gt, const int32_t* h, const
   if (B == 0) return ok();
   if (!Gt || !h || !T || !inst || !thr || !out_s || !out_de || !out_st)
     return fail(ARP_E_ARG, "arpeggio_ws: null pointer");
-  const int waves = R < 4 ? R : 4;
+  const int waves = arpeggio_waves(R, D);
   return hip_check(arp::launch_arpeggio_fx(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, 0, 0, out_s, (long long*)out_de,
                                            (long long*)out_st, waves, (hipStream_t)stream, thr, gate ? 1 : 0,
                                            out_used),

@@ -58,6 +58,8 @@ def parse():
     ap.add_argument('--with-chord', action='store_true')
     ap.add_argument('--no-batched', action='store_true', help='skip the mixed-size batch measurement')
     ap.add_argument('--batched-frames', type=int, default=1 << 17)
+    ap.add_argument('--no-64u', action='store_true', help='skip the 64-user correlated-channel measurement')
+    ap.add_argument('--frames-64u', type=int, default=1 << 16)
     ap.add_argument('--copy-probe', action='store_true', help='run only a plain device copy of known size (PMC scale)')
     return ap.parse_args()
 
@@ -147,7 +149,7 @@ def time_pipe(pipe, b, steps, warmup, min_warm_s=1.0):
     B = pipe.B
     return dict(value=B * steps / dt, ms_per_step=dt / steps * 1e3, gpu_ms_per_step=g0.elapsed_time(g1) / steps,
                 anneal_ms=float(np.mean([a.elapsed_time(c) for a, c in ev])),
-                ber=float(errs.item()) / (B * 4 * NT * steps))
+                ber=float(errs.item()) / (B * 2 * pipe.D * steps))
 
 
 def cpu_baseline(b, pipe, B_cpu, args):
@@ -245,6 +247,25 @@ def batched(args):
                 ber=det.bit_errors() / nbits(fr), R=8, passes=64, snr_db=[8.0, 24.0], graph=True)
 
 
+def users64(args):
+    """64 users x 64 rx, 16-QAM, Kronecker exponential correlation rho = 0.5 at both ends (GPU generator), 20 dB,
+    2^16 frames, 32 anneals per frame: plain Arpeggio at 64 and 128 passes. K timed steps each after warm-up."""
+    import torch
+    from arpeggio_amd import frames
+    from arpeggio_amd.detector import DetectPipeline
+    B, nt = args.frames_64u, 64
+    b = frames.draw_device(args.seed + 64, 20.0, 20.0, B, nt, 16, rho_rx=0.5, rho_tx=0.5)
+    out = {}
+    for name, kw in (('arpeggio_64', dict(policy='arpeggio', cycles=64, T0=0.1, T1=0.1, q=0.0)),
+                     ('arpeggio_128', dict(policy='arpeggio', cycles=128, T0=0.1, T1=0.1, q=0.0))):
+        pipe = DetectPipeline(B, 2 * nt, 2 * nt, 16, 32, key=key_of(args.seed), inst0=0, **kw)
+        t = time_pipe(pipe, b, args.steps, args.warmup, args.warm_seconds)
+        out[name] = dict(value=t['value'], ms_per_step=t['ms_per_step'], anneal_ms=t['anneal_ms'], ber=t['ber'])
+        del pipe
+    return dict(unit='detections/s', frames=B, users=nt, rx=nt, M=16, copies=32, snr_db=20.0, rho=0.5,
+                data='GPU generator, Kronecker-correlated i.i.d. Rayleigh', **out)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -315,6 +336,9 @@ def main():
     if not args.no_cpu:
         log('cpu baseline on %d frames' % args.cpu_frames)
         out['cpu_baseline'] = cpu_baseline(b, plain, min(args.cpu_frames, B), args)
+    if not args.no_64u:
+        log('64 users, correlated channel (%d frames)' % args.frames_64u)
+        out['users64'] = users64(args)
     if not args.no_batched:
         log('mixed-size batch (%d frames)' % args.batched_frames)
         out['batched'] = batched(args)

@@ -81,11 +81,11 @@ def test_front_end_golden(golden):
         assert np.abs(Gi - Gr).max() <= 1 and (Gi == Gr).mean() > 0.999
 
 
-@pytest.mark.parametrize('R,delta', [(16, True), (1, True), (33, True)])
-def test_decode_select_vs_oracle(orc, R, delta):
+@pytest.mark.parametrize('R,delta,nt', [(16, True, 32), (1, True, 32), (33, True, 32), (8, True, 64), (8, True, 16)])
+def test_decode_select_vs_oracle(orc, R, delta, nt):
     from arpeggio_amd.detector import decode_select, front_end
-    say('decode', R, delta)
-    M, nt, B = 16, 32, 48
+    say('decode', R, delta, nt)
+    M, B = 16, 48
     b = frames(nt, M, 18.0, B, seed=77)
     o = front_end(b['H'], b['y'], b['N0'], M, 1, want=('xg',))
     g = torch.Generator(device='cuda')
@@ -111,13 +111,15 @@ def test_decode_select_vs_oracle(orc, R, delta):
     assert (be.cpu().numpy() == be_o).all()
 
 
-@pytest.mark.parametrize('policy', ['arpeggio', 'chord'])
-def test_pipeline_chain_bit_exact(orc, policy):
-    """The whole chain (front end -> anneal -> decode) equals the oracle chain run on the GPU front end's integers."""
+@pytest.mark.parametrize('policy,nt,rho,R', [('arpeggio', 32, 0.0, 16), ('chord', 32, 0.0, 16),
+                                             ('arpeggio', 64, 0.5, 32), ('arpeggio', 16, 0.0, 8)])
+def test_pipeline_chain_bit_exact(orc, policy, nt, rho, R):
+    """The whole chain (front end -> anneal -> decode) equals the oracle chain run on the GPU front end's integers;
+    the 64-user case is the Kronecker-correlated channel (rho = 0.5 at both ends) of the 64 x 64 configuration."""
     from arpeggio_amd.detector import DetectPipeline
-    M, nt, B, R = 16, 32, 40, 16
-    say('pipeline', policy)
-    b = frames(nt, M, 20.0, B, seed=11)
+    M, B = 16, 40 if nt <= 32 else 24
+    say('pipeline', policy, nt, rho, R)
+    b = frames(nt, M, 20.0 if nt <= 32 else 24.0, B, seed=11, rho_rx=rho, rho_tx=rho)
     cyc = 64 if policy == 'arpeggio' else 128
     pipe = DetectPipeline(B, 2 * nt, 2 * nt, M, R, policy=policy, cycles=cyc, T0=0.1 if policy == 'arpeggio' else 0.3,
                           T1=0.1 if policy == 'arpeggio' else 0.2, q=0.0 if policy == 'arpeggio' else 0.4,
