@@ -237,6 +237,102 @@ __global__ __launch_bounds__(64) void decode_w64_kernel(const float* __restrict_
   }
 }
 
+// Two-wave form for D, Mr <= 128 (up to 64 users): thread i keeps receive row i of H in registers, thread j forms
+// coordinate j of every candidate into LDS, and each candidate's row terms meet in a butterfly per wave and then
+// across the two waves through LDS.
+__global__ __launch_bounds__(128) void decode_w128_kernel(const float* __restrict__ H, const float* __restrict__ y,
+                                                          const int8_t* __restrict__ xg,
+                                                          const int8_t* __restrict__ spins, int R, int P, int D,
+                                                          int Mr, int M, int delta, int vec4,
+                                                          const int8_t* __restrict__ ltrue, int8_t* __restrict__ lhat,
+                                                          int32_t* __restrict__ biterr, float* __restrict__ dist_out) {
+  constexpr int DP = 128;
+  extern __shared__ float sm[];
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int C = R + 1;
+  float* Lc = sm;              // C x DP candidate levels
+  float* dsum = Lc + C * DP;   // C x 2 per-wave partial distances
+  const float* Hb = H + (size_t)b * Mr * D;
+  const bool row = t < Mr;
+  float hr[DP];
+  if (vec4) {
+#pragma unroll
+    for (int q = 0; q < DP / 4; ++q) {
+      floatx4 v = {0.f, 0.f, 0.f, 0.f};
+      if (row && 4 * q < D) v = *reinterpret_cast<const floatx4*>(Hb + (size_t)t * D + 4 * q);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) hr[4 * q + e] = v[e];
+    }
+  } else {
+#pragma unroll
+    for (int jj = 0; jj < DP; ++jj) hr[jj] = (row && jj < D) ? Hb[(size_t)t * D + jj] : 0.f;
+  }
+  const float yi = row ? y[(size_t)b * Mr + t] : 0.f;
+  const float lmax = M == 4 ? 1.f : 3.f;
+  const int N = delta ? P * D : (M == 4 ? D : 2 * D);
+  const bool col = t < D;
+  const int x0 = col ? (int)xg[(size_t)b * D + t] : 0;
+  for (int r = 0; r < C; ++r) {
+    float l = 0.f;
+    if (col) {
+      if (r == R) {
+        l = (float)x0;
+      } else {
+        const int8_t* s = spins + ((size_t)b * R + r) * N;
+        if (delta) {
+          int d = 0;
+          for (int p = 0; p < P; ++p) d += s[p * D + t];
+          l = fminf(fmaxf((float)(x0 + d), -lmax), lmax);
+        } else {
+          l = M == 4 ? (float)s[t] : (float)(2 * s[t] + s[D + t]);
+        }
+      }
+    }
+    Lc[r * DP + t] = l;
+  }
+  __syncthreads();
+  for (int r = 0; r < C; ++r) {
+    const float* lc = Lc + r * DP;
+    float sacc = 0.f;
+#pragma unroll
+    for (int jj = 0; jj < DP; jj += 4) {
+      const floatx4 lv = *reinterpret_cast<const floatx4*>(lc + jj);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (jj + e < D) sacc = __builtin_fmaf(hr[jj + e], lv[e], sacc);
+    }
+    const float e = yi - sacc;
+    float part = row ? e * e : 0.f;
+    for (int o = 32; o; o >>= 1) part += __shfl_xor(part, o);
+    if (lane == 0) dsum[2 * r + wv] = part;
+  }
+  __syncthreads();
+  int best = 0;
+  float bd = 0.f;
+  for (int r = 0; r < C; ++r) {
+    const float dr = dsum[2 * r] + dsum[2 * r + 1];
+    if (r == 0 || dr < bd) {
+      bd = dr;
+      best = r;
+    }
+  }
+  if (dist_out)
+    for (int r = t; r < C; r += 128) dist_out[(size_t)b * C + r] = dsum[2 * r] + dsum[2 * r + 1];
+  int errs = 0;
+  if (col) {
+    const int l = (int)Lc[best * DP + t];
+    if (lhat) lhat[(size_t)b * D + t] = (int8_t)l;
+    if (ltrue) errs = __builtin_popcount(gray_bits(l, M) ^ gray_bits(ltrue[(size_t)b * D + t], M));
+  }
+  if (biterr && ltrue) {
+    for (int o = 32; o; o >>= 1) errs += __shfl_xor(errs, o);
+    __syncthreads();
+    if (lane == 0) dsum[wv] = (float)errs;
+    __syncthreads();
+    if (t == 0) biterr[b] = (int)dsum[0] + (int)dsum[1];
+  }
+}
+
 size_t decode_lds_bytes(int R, int D, int Mr) {
   return sizeof(float) * ((size_t)Mr * (D + 1) + Mr + (size_t)(R + 1) * D + (R + 1 > 4 ? R + 1 : 4));
 }
@@ -259,6 +355,15 @@ hipError_t launch_decode_select(const float* H, const float* y, const int8_t* xg
     else if (DP == 48) ARP_DECODE_W64(48);
     else ARP_DECODE_W64(64);
 #undef ARP_DECODE_W64
+    return hipGetLastError();
+  }
+  if (D <= 128 && Mr <= 128) {
+    const size_t l128 = sizeof(float) * ((size_t)(R + 1) * 128 + 2 * (size_t)(R + 1) + 2);
+    if (l128 > 65536)
+      hipFuncSetAttribute((const void*)decode_w128_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l128);
+    const int vec4 = (D % 4 == 0) && ((reinterpret_cast<uintptr_t>(H) & 15) == 0);
+    hipLaunchKernelGGL(decode_w128_kernel, dim3(B), dim3(128), l128, st, H, y, xg, spins, R, P, D, Mr, M, delta, vec4,
+                       ltrue, lhat, biterr, dist_out);
     return hipGetLastError();
   }
   const size_t lds = decode_lds_bytes(R, D, Mr);
