@@ -32,6 +32,7 @@ SIGNATURES = {
     'arp_anneal_arpeggio_ws': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _P, _i32, _P, _P, _P, _P,
                                _P],
     'arp_anneal_arpeggio_batched': [_P, _i32, _i32, _u32, _u32, _P, _i32, _i32, _P],
+    'arp_mlp_start_fused': [_P, _i32, _i32, _P, _P, _P, _P, _P, _P, _i32, _i32, _f, _P, _P],
     'arp_mlp_gemm': [_P, _l, _l, _P, _l, _l, _i32, _i32, _i32, _P, _i32, _P, _P, _P, _l, _P],
     'arp_mlp_gelu_bwd': [_P, _P, _l, _P, _P, _P],
     'arp_mlp_bce': [_P, _l, _P, _i32, _i32, _P, _P, _P, _P],

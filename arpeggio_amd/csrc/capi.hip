@@ -237,6 +237,20 @@ extern "C" int arp_anneal_arpeggio_batched(const arp_batch_class* classes, int n
                    "arpeggio_batched launch");
 }
 
+extern "C" int arp_mlp_start_fused(const uint16_t* X, int M, int F, const uint16_t* W1t, const float* b1,
+                                   const uint16_t* W2t, const float* b2, const uint16_t* W3t, const float* b3, int H,
+                                   int N, float inv_temp, uint32_t* thr, void* stream) {
+  if (M < 0 || F < 32 || F % 32 || F > 504)
+    return fail(ARP_E_ARG, "mlp_start_fused: F = %d, need a multiple of 32 in 32..504 (M = %d)", F, M);
+  if (H != 256) return fail(ARP_E_ARG, "mlp_start_fused: hidden width %d, the fused kernel is built for 256", H);
+  if (N < 64 || N > 256 || N % 64) return fail(ARP_E_ARG, "mlp_start_fused: N = %d, need 64, 128, 192 or 256", N);
+  if (M == 0) return ok();
+  if (!X || !W1t || !b1 || !W2t || !b2 || !W3t || !b3 || !thr) return fail(ARP_E_ARG, "mlp_start_fused: null pointer");
+  return hip_check(arp::launch_mlp_start_fused(X, M, F, W1t, b1, W2t, b2, W3t, b3, N, inv_temp, thr,
+                                               (hipStream_t)stream),
+                   "mlp_start_fused launch");
+}
+
 extern "C" int arp_mlp_gemm(const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, int M, int N,
                             int K, const float* bias, int act, float* Cf, float* Cpre, uint16_t* Cb, long ldc,
                             void* stream) {

@@ -79,6 +79,9 @@ hipError_t launch_colsum(const float* X, long ld, int M, int N, float* out, hipS
 hipError_t launch_adam(float* W, const float* G, float* m, float* v, long n, float lr, float b1, float b2, float eps,
                        int step, uint16_t* Wb, hipStream_t st);
 hipError_t launch_cast_bf16(const float* x, long n, uint16_t* y, hipStream_t st);
+hipError_t launch_mlp_start_fused(const uint16_t* X, int M, int F, const uint16_t* W1t, const float* b1,
+                                  const uint16_t* W2t, const float* b2, const uint16_t* W3t, const float* b3, int N,
+                                  float inv_temp, uint32_t* thr, hipStream_t st);
 hipError_t launch_start_thr(const float* z, long ld, int M, int N, float inv_temp, uint32_t* thr, hipStream_t st);
 
 }  // namespace arp

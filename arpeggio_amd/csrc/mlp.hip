@@ -186,6 +186,109 @@ __global__ void start_thr_kernel(const float* __restrict__ z, long ld, int M, in
   thr[i] = (uint32_t)t;
 }
 
+// Fused inference: features -> thresholds for a tile of 64 rows in one workgroup. The tile's features are staged in
+// LDS once; each hidden layer's 64 x 256 output is computed by the four waves (64 columns each) from the LDS
+// activations and n-major weights read from L2 (one 16-byte load per lane per 32-wide k-step), then written back
+// over the activations as bf16 after GELU; the last layer's logits go straight to start thresholds. Every output
+// element accumulates the same bf16 products in the same order as gemm_bf16_kernel (k ascending in steps of 32, the
+// same lane split of k), so the thresholds equal the three-GEMM path's bit for bit.
+template <int NC>
+__device__ __forceinline__ void mlp_tile_layer(const uint16_t* act, int lda, int K, const uint16_t* __restrict__ Wt,
+                                               int n0, floatx4 (&acc)[4][NC], int li, int lg) {
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < NC; ++ct) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    bf16x8 a[4], bw[NC];
+#pragma unroll
+    for (int ct = 0; ct < NC; ++ct)
+      bw[ct] = *reinterpret_cast<const bf16x8*>(Wt + (size_t)(n0 + ct * 16 + li) * K + k0 + lg * 8);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) a[rt] = *reinterpret_cast<const bf16x8*>(act + (rt * 16 + li) * lda + k0 + lg * 8);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < NC; ++ct) acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], bw[ct], acc[rt][ct], 0, 0, 0);
+  }
+}
+
+template <int NC3>
+__global__ __launch_bounds__(256) void mlp_start_fused_kernel(const uint16_t* __restrict__ X, int M, int F,
+                                                              const uint16_t* __restrict__ W1t,
+                                                              const float* __restrict__ b1,
+                                                              const uint16_t* __restrict__ W2t,
+                                                              const float* __restrict__ b2,
+                                                              const uint16_t* __restrict__ W3t,
+                                                              const float* __restrict__ b3, int N, float inv_temp,
+                                                              uint32_t* __restrict__ thr) {
+  constexpr int HID = 256, TM = 64;
+  extern __shared__ __align__(16) uint16_t act[];
+  const int lda = (F > HID ? F : HID) + 8;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, li = lane & 15, lg = lane >> 4;
+  const int m0 = blockIdx.x * TM;
+  const int cpr = F / 8;
+  for (int x = tid; x < TM * cpr; x += 256) {
+    const int r = x / cpr, c = x - r * cpr;
+    uint4 v = {0u, 0u, 0u, 0u};
+    if (m0 + r < M) v = *reinterpret_cast<const uint4*>(X + (size_t)(m0 + r) * F + 8 * c);
+    *reinterpret_cast<uint4*>(act + r * lda + 8 * c) = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+    floatx4 acc[4][4];
+    const int n0 = wv * 64;
+    mlp_tile_layer<4>(act, lda, layer == 0 ? F : HID, layer == 0 ? W1t : W2t, n0, acc, li, lg);
+    const float* bb = layer == 0 ? b1 : b2;
+    __syncthreads();  // every wave has read the layer's input
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        const int col = n0 + ct * 16 + li;
+        const float bn = bb[col];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) act[(rt * 16 + lg * 4 + v) * lda + col] = f2bf(gelu_f(acc[rt][ct][v] + bn));
+      }
+    __syncthreads();
+  }
+  floatx4 acc[4][NC3];
+  const int n0 = wv * NC3 * 16;
+  mlp_tile_layer<NC3>(act, lda, HID, W3t, n0, acc, li, lg);
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < NC3; ++ct) {
+      const int col = n0 + ct * 16 + li;
+      const float bn = b3[col];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + rt * 16 + lg * 4 + v;
+        if (m >= M) continue;
+        const double p = 1.0 / (1.0 + exp(-(double)(acc[rt][ct][v] + bn) * (double)inv_temp));
+        double t = rint(p * 4294967296.0);
+        t = t < 1.0 ? 1.0 : (t > 4294967295.0 ? 4294967295.0 : t);
+        thr[(size_t)m * N + col] = (uint32_t)t;
+      }
+    }
+}
+
+hipError_t launch_mlp_start_fused(const uint16_t* X, int M, int F, const uint16_t* W1t, const float* b1,
+                                  const uint16_t* W2t, const float* b2, const uint16_t* W3t, const float* b3, int N,
+                                  float inv_temp, uint32_t* thr, hipStream_t st) {
+  const int lda = (F > 256 ? F : 256) + 8;
+  const size_t lds = sizeof(uint16_t) * 64 * (size_t)lda;
+  const dim3 grid((M + 63) / 64);
+  switch (N / 64) {
+    case 1: hipLaunchKernelGGL((mlp_start_fused_kernel<1>), grid, dim3(256), lds, st, X, M, F, W1t, b1, W2t, b2, W3t, b3, N, inv_temp, thr); break;
+    case 2: hipLaunchKernelGGL((mlp_start_fused_kernel<2>), grid, dim3(256), lds, st, X, M, F, W1t, b1, W2t, b2, W3t, b3, N, inv_temp, thr); break;
+    case 3: hipLaunchKernelGGL((mlp_start_fused_kernel<3>), grid, dim3(256), lds, st, X, M, F, W1t, b1, W2t, b2, W3t, b3, N, inv_temp, thr); break;
+    default: hipLaunchKernelGGL((mlp_start_fused_kernel<4>), grid, dim3(256), lds, st, X, M, F, W1t, b1, W2t, b2, W3t, b3, N, inv_temp, thr); break;
+  }
+  return hipGetLastError();
+}
+
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 hipError_t launch_gemm_bf16(const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn, int M, int N,

@@ -144,11 +144,27 @@ class WarmStartMLP:
             _lib.check(self.lib.arp_mlp_adam(p(W), p(grads[k]), p(m), p(v), W.numel(), lr, b1, b2, eps,
                                              self.step_count, p(wb), s), 'adam')
 
-    def thresholds(self, X, inv_temp=1.0, out=None):
-        """Per-spin start thresholds (M, N) uint32 (as int32 storage) from the features."""
-        Z = self.forward(X)
-        M, N = Z.shape
+    def _weights_nmajor(self):
+        """bf16 weights transposed to n-major (the fused kernel's operand layout), rebuilt after optimiser steps."""
+        if getattr(self, '_wt_step', None) != self.step_count or getattr(self, '_wt_src', None) is not self.wb:
+            self._wt = {k: self.wb[k].t().contiguous() for k in ('W1', 'W2', 'W3')}
+            self._wt_step, self._wt_src = self.step_count, self.wb
+        return self._wt
+
+    def thresholds(self, X, inv_temp=1.0, out=None, fused=True):
+        """Per-spin start thresholds (M, N) uint32 (as int32 storage) from the features: one fused kernel
+        (arp_mlp_start_fused), or the three GEMMs and the threshold kernel (fused=False); both give the same bits."""
+        M, N = X.shape[0], self.N
         thr = out if out is not None else torch.empty(M, N, dtype=torch.int32, device=X.device)
+        if fused and self.H == 256 and N % 64 == 0 and N <= 256 and self.F <= 504:
+            W = self._weights_nmajor()
+            P_ = self.params
+            p = _lib.ptr
+            _lib.check(self.lib.arp_mlp_start_fused(p(X), M, self.F, p(W['W1']), p(P_['b1']), p(W['W2']), p(P_['b2']),
+                                                    p(W['W3']), p(P_['b3']), self.H, N, float(inv_temp), p(thr),
+                                                    _lib.stream_ptr()), 'mlp_start_fused')
+            return thr
+        Z = self.forward(X)
         _lib.check(self.lib.arp_start_thresholds(_lib.ptr(Z), N, M, N, float(inv_temp), _lib.ptr(thr),
                                                  _lib.stream_ptr()), 'start_thresholds')
         return thr

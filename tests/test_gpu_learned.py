@@ -179,3 +179,18 @@ def test_trained_start_gains_over_random_start():
     assert hi < 1.0
     assert out['learned'][1] < 0.6 * out['rand'][1]
     assert out['learned'][2] > 0.9
+
+
+@pytest.mark.parametrize('M', [1, 63, 64, 1000])
+def test_fused_inference_equals_gemm_path(M):
+    """The fused features -> thresholds kernel gives the same bits as the three GEMMs + threshold kernel."""
+    from arpeggio_amd.learned import DEFAULT_WEIGHTS, WarmStartMLP
+    say('fused', M)
+    model = WarmStartMLP.load(DEFAULT_WEIGHTS)
+    rs = np.random.default_rng(M)
+    X = torch.as_tensor(bf(rs.normal(size=(M, model.F)) * 0.7)).to(torch.bfloat16).cuda()
+    X[:, 4 * model.D + 1:] = 0
+    a = model.thresholds(X, 0.35, fused=True)
+    b = model.thresholds(X, 0.35, fused=False)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
