@@ -16,7 +16,7 @@
 namespace arp {
 
 template <int MPL, bool LDSJ>
-__global__ __launch_bounds__(256) void chord_fx_kernel(const int16_t* __restrict__ Jg, const int32_t* __restrict__ hg,
+__global__ __launch_bounds__(1024) void chord_fx_kernel(const int16_t* __restrict__ Jg, const int32_t* __restrict__ hg,
                                                        const int32_t* __restrict__ h0g, const int32_t* __restrict__ Tq,
                                                        const int32_t* __restrict__ Cq, int q, int S, int P, int R,
                                                        int N, uint32_t k0, uint32_t k1, int inst0, int mode,
@@ -26,11 +26,11 @@ __global__ __launch_bounds__(256) void chord_fx_kernel(const int16_t* __restrict
   const int b = blockIdx.x;
   const size_t NN = (size_t)N * N;
   const int16_t* Jb = Jg + (size_t)b * NN;
-  // LDS: [J (N*N int16, padded to 16 B)] [red: 4 waves x (e lo, e hi, r)]
+  // LDS: [J (N*N int16, padded to 16 B)] [red: up to 16 waves x (e, r)]
   const size_t jbytes = LDSJ ? ((NN * 2 + 15) & ~(size_t)15) : 0;
   int16_t* J = LDSJ ? reinterpret_cast<int16_t*>(smem) : const_cast<int16_t*>(Jb);
   long long* red_e = reinterpret_cast<long long*>(smem + jbytes);
-  int* red_r = reinterpret_cast<int*>(smem + jbytes + 8 * 8);
+  int* red_r = reinterpret_cast<int*>(smem + jbytes + 16 * 8);
   if (LDSJ) {
     // N % 8 == 0 is guaranteed by the host (N is a multiple of 32), so N*N int16 is a whole number of 16 B words
     const int4* src = reinterpret_cast<const int4*>(Jb);
@@ -229,7 +229,7 @@ hipError_t launch_chord_fx(const int16_t* J, const int32_t* h, const int32_t* h0
                            int inst0, int mode, int32_t* out_e, int8_t* out_s, unsigned long long* stats, int waves,
                            size_t max_lds, hipStream_t st) {
   const size_t jb = (((size_t)N * N * 2 + 15) & ~(size_t)15);
-  const size_t red = 8 * 8 + 8 * 4;
+  const size_t red = 16 * 8 + 16 * 4;
   const bool lds_j = jb + red <= max_lds;
   const size_t lds = (lds_j ? jb : 0) + red;
   const int mpl = (N + ARP_WAVE - 1) / ARP_WAVE;

@@ -98,7 +98,9 @@ extern "C" int arp_anneal_chord(const int16_t* J, const int32_t* h, const int32_
   if (mode != 0 && mode != 1) return fail(ARP_E_ARG, "chord: mode = %d, expected 0 or 1", mode);
   if (B == 0) return ok();
   if (!J || !h || !h0 || (S > 0 && (!Tq || !Cq)) || !out_e || !out_s) return fail(ARP_E_ARG, "chord: null pointer");
-  const int waves = R < 4 ? R : 4;
+  // a frame with N > 128 stages up to 128 KiB of J, one workgroup per CU: sixteen anneals per workgroup fill it
+  const int wmax = N > 128 ? 16 : 4;
+  const int waves = R < wmax ? R : wmax;
   return hip_check(arp::launch_chord_fx(J, h, h0, Tq, Cq, q, S, P, R, N, B, k0, k1, inst0, mode, out_e, out_s,
                                         (unsigned long long*)stats, waves, device_max_lds(), (hipStream_t)stream),
                    "chord launch");

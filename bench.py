@@ -249,7 +249,8 @@ def batched(args):
 
 def users64(args):
     """64 users x 64 rx, 16-QAM, Kronecker exponential correlation rho = 0.5 at both ends (GPU generator), 20 dB,
-    2^16 frames, 32 anneals per frame: plain Arpeggio at 64 and 128 passes. K timed steps each after warm-up."""
+    2^16 frames, 32 anneals per frame: plain Arpeggio at 64 and 128 passes, K timed steps each after warm-up; and
+    Chord at 256 steps (t 0.3 -> 0.2, q 0.4, kappa 0.3) on the first 4,096 of the frames, 3 timed steps."""
     import torch
     from arpeggio_amd import frames
     from arpeggio_amd.detector import DetectPipeline
@@ -262,6 +263,14 @@ def users64(args):
         t = time_pipe(pipe, b, args.steps, args.warmup, args.warm_seconds)
         out[name] = dict(value=t['value'], ms_per_step=t['ms_per_step'], anneal_ms=t['anneal_ms'], ber=t['ber'])
         del pipe
+    Bc = min(B, 4096)
+    bc = {k: (v[:Bc] if torch.is_tensor(v) and v.dim() > 0 and v.shape[0] == B else v) for k, v in b.items()}
+    pipe = DetectPipeline(Bc, 2 * nt, 2 * nt, 16, 32, key=key_of(args.seed), inst0=0, policy='chord', cycles=256,
+                          T0=0.3, T1=0.2, q=0.4, kappa=0.3)
+    t = time_pipe(pipe, bc, 3, 1, 0.0)
+    out['chord_256'] = dict(value=t['value'], ms_per_step=t['ms_per_step'], anneal_ms=t['anneal_ms'], ber=t['ber'],
+                            frames=Bc)
+    del pipe
     return dict(unit='detections/s', frames=B, users=nt, rx=nt, M=16, copies=32, snr_db=20.0, rho=0.5,
                 data='GPU generator, Kronecker-correlated i.i.d. Rayleigh', **out)
 
