@@ -32,6 +32,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X: 256 CUs x 4 SIMDs at 2.4 GHz (rocminfo); a wave64 vector instruction holds a SIMD for 4 clocks; LDS
+# delivers 128 B per clock per CU
+SIMDS, CLOCK_HZ = 256 * 4, 2.4e9
+LDS_PEAK_GBS = 128 * 256 * CLOCK_HZ / 1e9
 NT, M = 32, 16
 METRIC = 'detections/s at fixed BER (32u 16-QAM), learned-start vs plain annealer'
 
@@ -202,12 +206,31 @@ def roofline(pipe, t, B, L, R, learned):
         for rec in json.load(open(tp)).get('launches', []):
             if (rec.get('frames'), rec.get('passes'), rec.get('copies'), rec.get('learned')) == (B, L, R, learned):
                 traffic = rec.get('hbm_bytes_per_launch')
+    # the bound that holds: vector-instruction issue. Vector quad-cycles per anneal-pass come from the SQ counters
+    # of the same launch shape (SQ_ACTIVE_INST_VALU / anneal-passes); the SIMD cycles are this launch's own time.
+    issue = None
+    ip = os.path.join(ROOT, 'profiles', 'arpeggio_anneal_issue.json')
+    if os.path.exists(ip):
+        for rec in json.load(open(ip)).get('launches', []):
+            if rec.get('start') == ('learned' if learned else 'rand'):
+                q = rec['valu_quad_cycles_per_anneal_pass']
+                busy = 4 * q * B * R * L
+                avail = ms * 1e-3 * SIMDS * CLOCK_HZ
+                issue = dict(bound='valu issue', valu_quad_cycles_per_anneal_pass=q,
+                             salu_quad_cycles_per_anneal_pass=rec['salu_quad_cycles_per_anneal_pass'],
+                             achieved_simd_cycles=busy, available_simd_cycles=avail, frac=busy / avail,
+                             counters_at=dict(frames=rec['frames'], copies=rec['copies'], passes=rec['passes']),
+                             source='profiles/arpeggio_anneal_issue.json')
+    lds_gbs = lds / (ms * 1e-3) / 1e9
     return dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
                 traffic=traffic, kernel=kern + (' (learned start)' if learned else ''), algorithmic_bytes=hbm,
                 launch_ms=ms, flips_per_anneal=flips,
-                lds=dict(algorithmic_bytes=lds, achieved_GBs=lds / (ms * 1e-3) / 1e9),
-                note='the anneal is an on-chip loop bound by scalar/vector integer issue (Philox4x32-10, the 64-bit '
-                     'decision, the ballot scan), not by HBM or the matrix cores; bytes are the algorithmic ones')
+                lds=dict(algorithmic_bytes=lds, achieved_GBs=lds_gbs, peak_GBs=LDS_PEAK_GBS,
+                         frac=lds_gbs / LDS_PEAK_GBS),
+                issue=issue,
+                note='the anneal is an on-chip loop bound by vector-instruction issue (Philox4x32-10, the 64-bit '
+                     'decision, the ballot scan): issue.frac is the fraction of SIMD cycles spent issuing its vector '
+                     'instructions; HBM and LDS bytes are the algorithmic ones')
 
 
 def batched(args):
