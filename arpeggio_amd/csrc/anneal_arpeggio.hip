@@ -16,15 +16,23 @@
 
 namespace arp {
 
+// (a ^ s) + b in one v_xad_u32 (s wave-uniform); the compiler splits the xor from the add when a subtraction follows
+__device__ __forceinline__ int32_t xad(int32_t a, int32_t s, int32_t b) {
+  int32_t r;
+  asm("v_xad_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(s), "v"(b));
+  return r;
+}
+
 // One anneal per wave. G4 is the frame's Gram column block in LDS pre-scaled by 4 (G4[j*D + i] = 4 G_ij), so a flip of
 // lane j with old spin s adds s*G4[j][i] to every w_i and 4 (s w_j + 2 G_jj) to de.
 // Spins: wave masks (bit l of Sm[p][c]: spin of copy p at coordinate c*64 + l is +1) while the start is built, then
 // per-lane signs (0 / -1) for the passes. Test: lane j of copy p flips iff q_j = s w_j + 2 G_jj <= tq_j, tq_j the
-// high word of one multiply-add per pass; q_j is two vector ops from the sign, and the committed lane's q_j is also
+// high word of one multiply-add per pass; q_j = (w_j ^ sg_j) + (2 G_jj - sg_j) is one xor-add from the sign mask sg_j
+// (0 / -1), and the committed lane's q_j is also
 // its energy step (de += 4 q_j), so a flip reads two lane values (q and the sign). A pass's flips gather in a scalar
 // mask that updates the signs, the flip count and the last-lane count once at the end of the pass. The scalar unit
 // and the vector unit each issue about one wave instruction per 4 cycles per SIMD, so the loop splits its work
-// between them (per committed flip about ten of each).
+// between them (per committed flip 7 vector and about 11 scalar instructions).
 // tests (the reference's comparator passes) = L + flips - flips on the last lane (arpeggio_ref59.py:60-71).
 // w_i = h_i - 2 sum_j G_ij m_j = h_i - sum_j G4_ij (m_j / 2) for the state in Sm (m_j = sum_p s_pj, even: P is even)
 template <int P, int CPL>
@@ -211,8 +219,10 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
             // fm collects the pass's flips; the signs take them at the end of the pass. The committed lane's old
             // spin is its start-of-pass sign, read from that lane.
             const int32_t sg = sgp[p][c];
+            // q = (w ^ sg) - sg + dj2 = (w ^ sg) + e2: one xor-add per re-test
+            const int32_t e2 = dj2[c] - sg;
             uint64_t rem = vmask[c], fm = 0;  // rem: lanes the scan has not passed yet
-            int32_t qv = ((w[c] ^ sg) - sg) + dj2[c];
+            int32_t qv = (w[c] ^ sg) + e2;
             uint64_t m = ballot(qv <= tq[c]) & rem;
             while (m) {
               const int l = __builtin_ctzll(m);
@@ -220,10 +230,10 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
               const int32_t sl = readlane(sg, l);   // 0: old spin +1 (w += 4 G col), -1: old spin -1 (w -= ...)
               const int32_t* col = G4 + (c * ARP_WAVE + l) * D;
 #pragma unroll
-              for (int cc = 0; cc < CPL; ++cc) w[cc] += (col[cc * ARP_WAVE + lane] ^ sl) - sl;
+              for (int cc = 0; cc < CPL; ++cc) w[cc] = xad(col[cc * ARP_WAVE + lane], sl, w[cc]) - sl;
               mask_set(fm, l);
               rem &= (~1ull) << l;
-              qv = ((w[c] ^ sg) - sg) + dj2[c];
+              qv = (w[c] ^ sg) + e2;
               m = ballot(qv <= tq[c]) & rem;
             }
             const bool fl = __builtin_amdgcn_inverse_ballot_w64(fm);
