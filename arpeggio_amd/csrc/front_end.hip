@@ -344,23 +344,24 @@ __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restri
   // Sweep, in place. Step k: every lane publishes a_kj (its own a[k]); lane j != k forms r = a_kj / a_kk and
   // subtracts a_ik r from a_ij, then keeps r as a_kj; lane k takes r = 1 - 1/a_kk, which turns the same update into
   // a_ik -> a_ik / a_kk, and keeps -1/a_kk. The broadcast buffer alternates between two halves so that a step's
-  // store never meets the previous step's reads; both loops are fully unrolled so that a[k] and a[i] are fixed
-  // registers.
+  // store never meets the previous step's reads. All DP/4 broadcast loads of a step are issued before its first FMA
+  // (the scheduling fence keeps the compiler from sinking them to their uses, which serialises one LDS round trip per
+  // four FMAs); both loops are fully unrolled so that a[k] and a[i] are fixed registers.
 #pragma clang loop unroll(full)
   for (int k = 0; k < DP; ++k) {
     float* cb = colb + (k & 1) * 64;
     cb[lane] = a[k];
     __syncthreads();
-    const float p = __builtin_amdgcn_rcpf(cb[k]);
+    floatx4 f[DP / 4];
+#pragma unroll
+    for (int i = 0; i < DP / 4; ++i) f[i] = *reinterpret_cast<const floatx4*>(cb + 4 * i);
+    __builtin_amdgcn_sched_barrier(0);
+    const float p = __builtin_amdgcn_rcpf(f[k / 4][k % 4]);
     const bool is_k = lane == k;
     const float rk = is_k ? 1.f - p : a[k] * p;
 #pragma clang loop unroll(full)
-    for (int i = 0; i < DP; i += 4) {
-      const floatx4 f = *reinterpret_cast<const floatx4*>(cb + i);
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (i + v != k) a[i + v] = __builtin_fmaf(-f[v], rk, a[i + v]);
-    }
+    for (int i = 0; i < DP; ++i)
+      if (i != k) a[i] = __builtin_fmaf(-f[i / 4][i % 4], rk, a[i]);
     a[k] = is_k ? -p : rk;
   }
   __syncthreads();
