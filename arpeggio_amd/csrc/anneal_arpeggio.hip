@@ -219,21 +219,28 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
             // fm collects the pass's flips; the signs take them at the end of the pass. The committed lane's old
             // spin is its start-of-pass sign, read from that lane.
             const int32_t sg = sgp[p][c];
-            // q = (w ^ sg) - sg + dj2 = (w ^ sg) + e2: one xor-add per re-test
+            // q = (w ^ sg) - sg + dj2 = (w ^ sg) + e2: one xor-add per re-test. With one column block the field
+            // update is an xor-add too; with two or more, the plain form measures faster (the loads of the column
+            // blocks then issue together).
             const int32_t e2 = dj2[c] - sg;
             uint64_t rem = vmask[c], fm = 0;  // rem: lanes the scan has not passed yet
-            int32_t qv = (w[c] ^ sg) + e2;
+            int32_t qv = CPL == 1 ? (w[c] ^ sg) + e2 : ((w[c] ^ sg) - sg) + dj2[c];
             uint64_t m = ballot(qv <= tq[c]) & rem;
             while (m) {
               const int l = __builtin_ctzll(m);
               de4 += readlane(qv, l);
               const int32_t sl = readlane(sg, l);   // 0: old spin +1 (w += 4 G col), -1: old spin -1 (w -= ...)
               const int32_t* col = G4 + (c * ARP_WAVE + l) * D;
+              if constexpr (CPL == 1) {
+                w[0] = xad(col[lane], sl, w[0]) - sl;
+                qv = (w[0] ^ sg) + e2;
+              } else {
 #pragma unroll
-              for (int cc = 0; cc < CPL; ++cc) w[cc] = xad(col[cc * ARP_WAVE + lane], sl, w[cc]) - sl;
+                for (int cc = 0; cc < CPL; ++cc) w[cc] += (col[cc * ARP_WAVE + lane] ^ sl) - sl;
+                qv = ((w[c] ^ sg) - sg) + dj2[c];
+              }
               mask_set(fm, l);
               rem &= (~1ull) << l;
-              qv = (w[c] ^ sg) + e2;
               m = ballot(qv <= tq[c]) & rem;
             }
             const bool fl = __builtin_amdgcn_inverse_ballot_w64(fm);
