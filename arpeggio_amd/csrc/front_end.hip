@@ -577,11 +577,33 @@ __global__ __launch_bounds__(128) void front_end_w128_kernel(const float* __rest
       const float p = __builtin_amdgcn_rcpf(cbk[k]);
       const bool is_k = j == k;
       const float rk = is_k ? 1.f - p : bk * p;
+      // The 128 broadcasts in groups of 8 loads, software-pipelined: group g + 1 is in flight while group g's FMAs
+      // issue (the scheduling fences keep the compiler from sinking each load to its four FMAs).
+      const floatx4* c4 = reinterpret_cast<const floatx4*>(cbk);
+      floatx4 f0[8], f1[8];
 #pragma unroll
-      for (int i = 0; i < DP; i += 4) {
-        const floatx4 f = *reinterpret_cast<const floatx4*>(cbk + i);
+      for (int u = 0; u < 8; ++u) f0[u] = c4[u];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) a[i + v] = __builtin_fmaf(-f[v], rk, a[i + v]);
+      for (int g = 0; g < DP / 32; g += 2) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) f1[u] = c4[8 * (g + 1) + u];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) a[32 * g + 4 * u + v] = __builtin_fmaf(-f0[u][v], rk, a[32 * g + 4 * u + v]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (g + 2 < DP / 32) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) f0[u] = c4[8 * (g + 2) + u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            a[32 * (g + 1) + 4 * u + v] = __builtin_fmaf(-f1[u][v], rk, a[32 * (g + 1) + 4 * u + v]);
+        __builtin_amdgcn_sched_barrier(0);
       }
       const floatx4* fb = reinterpret_cast<const floatx4*>(cbk + 16 * kb);
 #pragma unroll
