@@ -214,11 +214,13 @@ def roofline(pipe, t, B, L, R, learned):
         for rec in json.load(open(ip)).get('launches', []):
             if rec.get('start') == ('learned' if learned else 'rand'):
                 q = rec['valu_quad_cycles_per_anneal_pass']
+                qs = rec['salu_quad_cycles_per_anneal_pass']
                 busy = 4 * q * B * R * L
                 avail = ms * 1e-3 * SIMDS * CLOCK_HZ
-                issue = dict(bound='valu issue', valu_quad_cycles_per_anneal_pass=q,
-                             salu_quad_cycles_per_anneal_pass=rec['salu_quad_cycles_per_anneal_pass'],
+                issue = dict(bound='instruction issue (vector and scalar)', valu_quad_cycles_per_anneal_pass=q,
+                             salu_quad_cycles_per_anneal_pass=qs,
                              achieved_simd_cycles=busy, available_simd_cycles=avail, frac=busy / avail,
+                             salu_frac=4 * qs * B * R * L / avail,
                              counters_at=dict(frames=rec['frames'], copies=rec['copies'], passes=rec['passes']),
                              source='profiles/arpeggio_anneal_issue.json')
     lds_gbs = lds / (ms * 1e-3) / 1e9
@@ -228,9 +230,10 @@ def roofline(pipe, t, B, L, R, learned):
                 lds=dict(algorithmic_bytes=lds, achieved_GBs=lds_gbs, peak_GBs=LDS_PEAK_GBS,
                          frac=lds_gbs / LDS_PEAK_GBS),
                 issue=issue,
-                note='the anneal is an on-chip loop bound by vector-instruction issue (Philox4x32-10, the 64-bit '
-                     'decision, the ballot scan): issue.frac is the fraction of SIMD cycles spent issuing its vector '
-                     'instructions; HBM and LDS bytes are the algorithmic ones')
+                note='the anneal is an on-chip loop bound by instruction issue (Philox4x32-10, the 64-bit decision, '
+                     'the ballot scan with its scalar find-first-set chain): issue.frac and issue.salu_frac are the '
+                     'fractions of SIMD cycles spent issuing its vector and scalar instructions; HBM and LDS bytes '
+                     'are the algorithmic ones')
 
 
 def batched(args):
