@@ -99,7 +99,9 @@ __device__ __forceinline__ long long start_energy(const int32_t* __restrict__ hb
   return e;
 }
 
-template <int P, int CPL, int MODE, bool FULL>
+// DS: D as a compile-time constant (0: D at run time); with it the column address l*D folds into the LDS address
+// shift and the lane masks into constants.
+template <int P, int CPL, int MODE, bool FULL, int DS = 0>
 __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb,
                                                 const int32_t* __restrict__ T, int L, int q, int D, uint32_t r,
                                                 uint32_t inst, uint32_t k0, uint32_t k1, int start,
@@ -107,6 +109,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
                                                 long long* __restrict__ out_st, size_t t,
                                                 const uint32_t* __restrict__ thr, int gate,
                                                 int8_t* __restrict__ used) {
+  if (DS) D = DS;
   const int lane = lane_id();
   bool valid[CPL];
   int32_t w[CPL], dj2[CPL];
@@ -309,7 +312,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
 
 // grid.x = B frames, block = 64 W threads. LDS: 4 Gt, D*D int32 (+ one wave of padding so that the lanes past D of
 // the last column stay inside the allocation).
-template <int P, int CPL, int MODE, bool FULL>
+template <int P, int CPL, int MODE, bool FULL, int DS>
 __global__ __launch_bounds__(512) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt, const int32_t* __restrict__ h,
                                                           const int32_t* __restrict__ T, int L, int q, int R, int D,
                                                           const uint32_t* __restrict__ inst, uint32_t k0, uint32_t k1,
@@ -319,6 +322,7 @@ __global__ __launch_bounds__(512) void arpeggio_fx_kernel(const int32_t* __restr
                                                           const uint32_t* __restrict__ thr, int gate,
                                                           int8_t* __restrict__ used) {
   extern __shared__ int32_t sG[];
+  if (DS) D = DS;
   const int b = blockIdx.x;
   const int nG = D * D;
   const int32_t* g = Gt + (size_t)b * nG;
@@ -327,19 +331,20 @@ __global__ __launch_bounds__(512) void arpeggio_fx_kernel(const int32_t* __restr
   const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
   const uint32_t in = inst[b];
   for (int r = wv; r < R; r += W)
-    arpeggio_anneal<P, CPL, MODE, FULL>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)r, in, k0, k1, start, out_s, out_de,
+    arpeggio_anneal<P, CPL, MODE, FULL, DS>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)r, in, k0, k1, start, out_s,
+                                            out_de,
                                   out_st, (size_t)b * R + r, thr ? thr + (size_t)b * P * D : nullptr, gate, used);
 }
 
-template <int P, int CPL, int MODE, bool FULL>
+template <int P, int CPL, int MODE, bool FULL, int DS>
 static void launch_pcmf(dim3 grid, dim3 block, size_t lds, hipStream_t st, const int32_t* Gt, const int32_t* h,
                         const int32_t* T, int L, int q, int R, int D, const uint32_t* inst, uint32_t k0, uint32_t k1,
                         int start, int8_t* out_s, long long* out_de, long long* out_st, const uint32_t* thr, int gate,
                         int8_t* used) {
   if (lds > 65536)
-    hipFuncSetAttribute((const void*)arpeggio_fx_kernel<P, CPL, MODE, FULL>,
+    hipFuncSetAttribute((const void*)arpeggio_fx_kernel<P, CPL, MODE, FULL, DS>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((arpeggio_fx_kernel<P, CPL, MODE, FULL>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0,
+  hipLaunchKernelGGL((arpeggio_fx_kernel<P, CPL, MODE, FULL, DS>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0,
                      k1, start, out_s, out_de, out_st, thr, gate, used);
 }
 
@@ -349,12 +354,23 @@ static void launch_pcm(dim3 grid, dim3 block, size_t lds, hipStream_t st, const 
                        const int32_t* T, int L, int q, int R, int D, const uint32_t* inst, uint32_t k0, uint32_t k1,
                        int start, int8_t* out_s, long long* out_de, long long* out_st, const uint32_t* thr, int gate,
                        int8_t* used) {
+  // 32 users (D = 64, two copies, first accept): D as a compile-time constant
+  constexpr bool STATIC64 = P == 2 && CPL == 1 && MODE == 0;
+  if (STATIC64 && D == 64) {
+    if (L % (4 * P) == 0)
+      launch_pcmf<P, CPL, MODE, true, STATIC64 ? 64 : 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start,
+                                                         out_s, out_de, out_st, thr, gate, used);
+    else
+      launch_pcmf<P, CPL, MODE, false, STATIC64 ? 64 : 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1,
+                                                          start, out_s, out_de, out_st, thr, gate, used);
+    return;
+  }
   if (L % (4 * P) == 0)
-    launch_pcmf<P, CPL, MODE, true>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de,
-                                    out_st, thr, gate, used);
+    launch_pcmf<P, CPL, MODE, true, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de,
+                                       out_st, thr, gate, used);
   else
-    launch_pcmf<P, CPL, MODE, false>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de,
-                                     out_st, thr, gate, used);
+    launch_pcmf<P, CPL, MODE, false, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de,
+                                        out_st, thr, gate, used);
 }
 
 template <int P>
