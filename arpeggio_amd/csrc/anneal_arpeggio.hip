@@ -430,7 +430,10 @@ __global__ __launch_bounds__(1024) void arpeggio_batched_kernel(ArpBatchArgs a, 
       const int32_t* hb = C.h + (size_t)b * D;
       const size_t at = (size_t)b * R + r;
       if (cpl == 1) {
-        if (full)
+        if (full && D == 64)
+          arpeggio_anneal<2, 1, 0, true, 64>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
+                                             C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+        else if (full)
           arpeggio_anneal<2, 1, 0, true>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
                                          C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
         else
