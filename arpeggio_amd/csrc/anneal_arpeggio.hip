@@ -187,7 +187,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
     const int n4 = kb / (4 * P);
     int32_t Tb[4 * P];
 #pragma unroll
-    for (int i = 0; i < 4 * P; ++i) Tb[i] = kb + i < L ? T[kb + i] : 0;
+    for (int i = 0; i < 4 * P; ++i) Tb[i] = (FULL || kb + i < L) ? T[kb + i] : 0;  // FULL: L % 4P == 0
 #pragma unroll
     for (int nm = 0; nm < 4; ++nm) {
 #pragma unroll
