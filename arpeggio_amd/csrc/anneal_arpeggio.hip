@@ -243,7 +243,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
                 qv = ((w[c] ^ sg) - sg) + dj2[c];
               }
               mask_set(fm, l);
-              rem &= (~1ull) << l;
+              rem = vmask[c] & ((~1ull) << l);  // the scan passes lanes in index order: the lanes above l remain
               m = ballot(qv <= tq[c]) & rem;
             }
             const bool fl = __builtin_amdgcn_inverse_ballot_w64(fm);
