@@ -354,15 +354,23 @@ static void launch_pcm(dim3 grid, dim3 block, size_t lds, hipStream_t st, const 
                        const int32_t* T, int L, int q, int R, int D, const uint32_t* inst, uint32_t k0, uint32_t k1,
                        int start, int8_t* out_s, long long* out_de, long long* out_st, const uint32_t* thr, int gate,
                        int8_t* used) {
-  // 32 users (D = 64, two copies, first accept): D as a compile-time constant
-  constexpr bool STATIC64 = P == 2 && CPL == 1 && MODE == 0;
-  if (STATIC64 && D == 64) {
-    if (L % (4 * P) == 0)
-      launch_pcmf<P, CPL, MODE, true, STATIC64 ? 64 : 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start,
-                                                         out_s, out_de, out_st, thr, gate, used);
+  // 16, 32 and 64 users (D = 32, 64, 128; two copies, first accept): D as a compile-time constant
+  constexpr bool STATIC = P == 2 && MODE == 0;
+  constexpr int DSA = STATIC ? (CPL == 1 ? 64 : 128) : 0, DSB = STATIC && CPL == 1 ? 32 : 0;
+  if (STATIC && (D == DSA || D == DSB)) {
+    const bool full = L % (4 * P) == 0;
+    if (D == DSA && full)
+      launch_pcmf<P, CPL, MODE, true, DSA>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s,
+                                           out_de, out_st, thr, gate, used);
+    else if (D == DSA)
+      launch_pcmf<P, CPL, MODE, false, DSA>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s,
+                                            out_de, out_st, thr, gate, used);
+    else if (full)
+      launch_pcmf<P, CPL, MODE, true, DSB>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s,
+                                           out_de, out_st, thr, gate, used);
     else
-      launch_pcmf<P, CPL, MODE, false, STATIC64 ? 64 : 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1,
-                                                          start, out_s, out_de, out_st, thr, gate, used);
+      launch_pcmf<P, CPL, MODE, false, DSB>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s,
+                                            out_de, out_st, thr, gate, used);
     return;
   }
   if (L % (4 * P) == 0)
@@ -433,6 +441,9 @@ __global__ __launch_bounds__(1024) void arpeggio_batched_kernel(ArpBatchArgs a, 
         if (full && D == 64)
           arpeggio_anneal<2, 1, 0, true, 64>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
                                              C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+        else if (full && D == 32)
+          arpeggio_anneal<2, 1, 0, true, 32>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
+                                             C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
         else if (full)
           arpeggio_anneal<2, 1, 0, true>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
                                          C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
@@ -440,7 +451,10 @@ __global__ __launch_bounds__(1024) void arpeggio_batched_kernel(ArpBatchArgs a, 
           arpeggio_anneal<2, 1, 0, false>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
                                           C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
       } else {
-        if (full)
+        if (full && D == 128)
+          arpeggio_anneal<2, 2, 0, true, 128>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
+                                              C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+        else if (full)
           arpeggio_anneal<2, 2, 0, true>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
                                          C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
         else
