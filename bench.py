@@ -200,7 +200,7 @@ def roofline(pipe, t, B, L, R, learned):
     flips = float(st[..., 0].mean())
     lds = B * R * 4 * D * (flips + D)
     traffic = None
-    kern = 'arpeggio_fx_kernel<2,1>'
+    kern = 'arpeggio_fx_kernel<2,1,0,true,64>'
     tp = os.path.join(ROOT, 'profiles', 'arpeggio_anneal_traffic.json')
     if os.path.exists(tp):
         for rec in json.load(open(tp)).get('launches', []):
