@@ -16,6 +16,18 @@
 
 namespace arp {
 
+// a < b for wave-uniform 64-bit values, on the scalar unit: the sign of the high word of a - b (no overflow within
+// the documented ranges)
+__device__ __forceinline__ bool s64_lt(long long a, long long b) {
+  uint32_t lo, hi;
+  asm("s_sub_u32 %0, %2, %4\n\ts_subb_u32 %1, %3, %5"
+      : "=&s"(lo), "=s"(hi)
+      : "s"((uint32_t)a), "s"((uint32_t)((unsigned long long)a >> 32)), "s"((uint32_t)b),
+        "s"((uint32_t)((unsigned long long)b >> 32))
+      : "scc");
+  return (int32_t)hi < 0;
+}
+
 // (a ^ s) + b in one v_xad_u32 (s wave-uniform); the compiler splits the xor from the add when a subtraction follows
 __device__ __forceinline__ int32_t xad(int32_t a, int32_t s, int32_t b) {
   int32_t r;
@@ -274,8 +286,11 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
             fc[c] += fl ? 1 : 0;
           }
         }
-        // best-of-trajectory, strict (the first minimum wins)
-        if (de4 < best4) {
+        // best-of-trajectory, strict (the first minimum wins). The test runs on the scalar unit (sign of the 64-bit
+        // difference) and the copy behind a wave-uniform branch: the empty asm keeps the compiler from turning the
+        // copy into vector selects on every pass.
+        if (s64_lt(de4, best4)) {
+          asm volatile("");
           best4 = de4;
 #pragma unroll
           for (int pp = 0; pp < P; ++pp)
