@@ -258,9 +258,12 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
               rem = vmask[c] & ((~1ull) << l);  // the scan passes lanes in index order: the lanes above l remain
               m = ballot(qv <= tq[c]) & rem;
             }
-            const bool fl = __builtin_amdgcn_inverse_ballot_w64(fm);
-            sgp[p][c] ^= fl ? -1 : 0;
-            fc[c] += fl ? 1 : 0;
+            if (fm) {  // passes without a flip skip the sign update (the empty asm keeps the branch)
+              asm volatile("");
+              const bool fl = __builtin_amdgcn_inverse_ballot_w64(fm);
+              sgp[p][c] ^= fl ? -1 : 0;
+              fc[c] += fl ? 1 : 0;
+            }
           }
         } else {
           // 'par': every lane of the pass decided on one snapshot, the flips then applied in index order
