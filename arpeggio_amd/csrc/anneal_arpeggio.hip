@@ -16,22 +16,6 @@
 
 namespace arp {
 
-// Philox4x32-10 (the same rounds as philox10_kv) with each round's two three-input XORs as one v_bitop3_b32 each
-// (truth table 0x96 = a ^ b ^ c)
-__device__ __forceinline__ void philox10_b3(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
-                                            const PhiloxKeys& K) {
-#pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
-    const uint32_t x0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, K.k0[i], 0x96);
-    const uint32_t x2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, K.k1[i], 0x96);
-    c0 = x0;
-    c1 = (uint32_t)p1;
-    c2 = x2;
-    c3 = (uint32_t)p0;
-  }
-}
-
 // a < b for wave-uniform 64-bit values, on the scalar unit: the sign of the high word of a - b (no overflow within
 // the documented ranges)
 __device__ __forceinline__ bool s64_lt(long long a, long long b) {
@@ -226,7 +210,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             uint32_t c0 = (uint32_t)n4, c1 = (uint32_t)(p * D + c * ARP_WAVE + lane), c2 = r, c3 = inst;
-            philox10_b3(c0, c1, c2, c3, K);
+            philox10_kv(c0, c1, c2, c3, K);
             rw[p][c][0] = c0;
             rw[p][c][1] = c1;
             rw[p][c][2] = c2;

@@ -20,7 +20,9 @@ __device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c
     const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    const uint32_t x0 = hi1 ^ c1 ^ k0, x2 = hi0 ^ c3 ^ k1;
+    // each three-input XOR is one v_bitop3_b32 (truth table 0x96 = a ^ b ^ c)
+    const uint32_t x0 = __builtin_amdgcn_bitop3_b32(hi1, c1, k0, 0x96);
+    const uint32_t x2 = __builtin_amdgcn_bitop3_b32(hi0, c3, k1, 0x96);
     c0 = x0;
     c1 = lo1;
     c2 = x2;
@@ -54,7 +56,8 @@ __device__ __forceinline__ void philox10_kv(uint32_t& c0, uint32_t& c1, uint32_t
     const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    const uint32_t x0 = hi1 ^ c1 ^ K.k0[i], x2 = hi0 ^ c3 ^ K.k1[i];
+    const uint32_t x0 = __builtin_amdgcn_bitop3_b32(hi1, c1, K.k0[i], 0x96);
+    const uint32_t x2 = __builtin_amdgcn_bitop3_b32(hi0, c3, K.k1[i], 0x96);
     c0 = x0;
     c1 = lo1;
     c2 = x2;
