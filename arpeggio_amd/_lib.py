@@ -23,6 +23,7 @@ class BatchClass(ctypes.Structure):
 
 SIGNATURES = {
     'arp_philox_kat': [_P, _P, _i32, _P],
+    'arp_set_anneal_waves': [_i32],
     'arp_anneal_arpeggio': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _i32, _P, _P, _P,
                             _P],
     'arp_anneal_chord': [_P, _P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _i32, _P, _P, _P,
@@ -45,6 +46,11 @@ SIGNATURES = {
     'arp_anneal_chord_f32': [_P, _P, _P, _P, _P, ctypes.c_float, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _P, _P,
                              _P, _P, _P],
     'arp_decode_select': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _P, _P, _P, _P, _P],
+    'arp_anneal_arpeggio_list': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _P, _i32, _i32,
+                                 _P, _P, _P, _P, _P, _P, _P],
+    'arp_decode_select_list': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _P, _P, _P, _P, _P, _P, _P,
+                               _P, _P],
+    'arp_certify': [_P, _P, _f, _f, _i32, _P, _P, _P],
 }
 
 _lib = None

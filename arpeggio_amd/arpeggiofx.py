@@ -31,6 +31,18 @@ def _range_check(G, h, P, T, q):
         raise ValueError('arpeggio: 4T + 2|G_jj| + |q| + 2T exceeds the int32 decision range')
 
 
+def schedule_check(T, q, P, D, Q=4096):
+    """The ranges of _range_check for couplings the front end writes (|G_ij|, |h_i| <= Q after its scaling): a check of
+    the schedule alone, for pipelines whose integer Gram is produced on the device."""
+    Tl = [int(t) for t in T]
+    if min(Tl) < 0 or max(Tl) >= 2 ** 29:
+        raise ValueError('arpeggio: pass temperatures must lie in [0, 2^29), got [%d, %d]' % (min(Tl), max(Tl)))
+    wmax = int(Q) * (1 + 2 * P * D)
+    if wmax + 2 * int(Q) + abs(int(q)) + 2 * max(Tl) >= INT31 or 6 * max(Tl) + 2 * int(Q) + abs(int(q)) >= INT31:
+        raise ValueError('arpeggio: schedule (max T %d, q %d) exceeds the int32 field range at D = %d' %
+                         (max(Tl), q, D))
+
+
 def solve_device(G, h, P, R, T, q, inst, key, start, mode='arpeggio', transposed=False):
     """Device form. G (B, D, D) integer Gram (G[b, i, j] = G_ij; pass transposed=True if it is already Gt), h (B, D),
     T (L,) integers, inst (B,) integers < 2^32, key 64-bit, start 'rand'/'bal'. Returns device tensors

@@ -19,6 +19,7 @@ import math
 import torch
 
 from . import _lib
+from .arpeggiofx import schedule_check
 from .detector import MASK, QSCALE, _n0_arg, arpeggio_schedule
 from .frames import PAM, complex_to_real
 
@@ -75,6 +76,7 @@ class BatchedDetector:
         self.R, self.P, self.K, self.Q, self.key = R, 2, 1, float(Q), key
         self.lib = _lib.load()
         T, self.q = arpeggio_schedule(passes // 2, T0, T1, q)
+        schedule_check(T, self.q, self.P, max(2 * c['nt'] for c in classes), Q)
         self.T = torch.tensor(T, dtype=torch.int32, device='cuda')
         self.L = len(T)
         self.cls = []

@@ -1,0 +1,40 @@
+// Optional arguments of the first-accept and decode kernels for frame-list (compacted) launches, and the launch
+// wrappers of the early-stop path (certificate + compaction). Included by the kernels that take them and by the C ABI.
+#pragma once
+#include "common.h"
+
+namespace arp {
+
+// Optional arguments of arpeggio_fx_kernel: learned-start thresholds (B, P D) with the energy gate and the kept flags,
+// a frame list (slot i anneals frame list[i] for i < *count), and the first Philox stream index r0.
+struct AnnealExtra {
+  const uint32_t* thr;
+  int8_t* used;
+  const int32_t* list;
+  const int32_t* count;
+  int gate;
+  int r0;
+};
+
+// Optional arguments of the decode kernels: a frame list (as AnnealExtra; spins are read from slot i, everything else
+// from frame list[i]), a prior candidate per frame (levels and distance) that a new candidate must beat strictly, and
+// the chosen candidate's distance out.
+struct DecodeExtra {
+  const int32_t* list;
+  const int32_t* count;
+  const int8_t* prior_l;
+  const float* prior_d;
+  float* best_d;
+};
+
+hipError_t launch_arpeggio_ex(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R, int D,
+                              int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode,
+                              int8_t* out_s, long long* out_de, long long* out_st, int waves, const AnnealExtra& ex,
+                              hipStream_t st);
+hipError_t launch_decode_ex(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R, int P,
+                            int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat, int32_t* biterr,
+                            float* dist_out, const DecodeExtra& ex, hipStream_t st);
+hipError_t launch_certify(const float* best_d, const float* N0v, float N0s, float thr_scale, int B, int32_t* list,
+                          int32_t* count, hipStream_t st);
+
+}  // namespace arp

@@ -13,6 +13,8 @@
 // Each (lane, copy) Philox block gives the words of 4 consecutive sweeps, so a block is computed every 4th sweep of
 // a copy instead of once per test.
 #include "kernels.h"
+#include "launch_bounds.h"
+#include "adaptive.h"
 
 namespace arp {
 
@@ -328,91 +330,89 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
   }
 }
 
-// grid.x = B frames, block = 64 W threads. LDS: 4 Gt, D*D int32 (+ one wave of padding so that the lanes past D of
-// the last column stay inside the allocation).
+// grid.x = B frames (or list slots), block = 64 W threads. LDS: 4 Gt, D*D int32 (+ one wave of padding so that the
+// lanes past D of the last column stay inside the allocation). With a frame list, workgroup i anneals frame list[i]
+// if i < *count and exits otherwise (a uniform exit before any barrier); its outputs go to slot i. Anneal rr of a
+// workgroup uses Philox stream r0 + rr, so a later launch with r0 = R continues a frame's anneals with new streams.
 template <int P, int CPL, int MODE, bool FULL, int DS>
-__global__ __launch_bounds__(512) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt, const int32_t* __restrict__ h,
-                                                          const int32_t* __restrict__ T, int L, int q, int R, int D,
-                                                          const uint32_t* __restrict__ inst, uint32_t k0, uint32_t k1,
-                                                          int start, int8_t* __restrict__ out_s,
-                                                          long long* __restrict__ out_de,
-                                                          long long* __restrict__ out_st,
-                                                          const uint32_t* __restrict__ thr, int gate,
-                                                          int8_t* __restrict__ used) {
+__global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt,
+                                                                               const int32_t* __restrict__ h,
+                                                                               const int32_t* __restrict__ T, int L,
+                                                                               int q, int R, int D,
+                                                                               const uint32_t* __restrict__ inst,
+                                                                               uint32_t k0, uint32_t k1, int start,
+                                                                               int8_t* __restrict__ out_s,
+                                                                               long long* __restrict__ out_de,
+                                                                               long long* __restrict__ out_st,
+                                                                               AnnealExtra ex) {
   extern __shared__ int32_t sG[];
   if (DS) D = DS;
-  const int b = blockIdx.x;
+  const int slot = blockIdx.x;
+  int b = slot;
+  if (ex.list) {
+    if (slot >= *ex.count) return;
+    b = ex.list[slot];
+  }
   const int nG = D * D;
   const int32_t* g = Gt + (size_t)b * nG;
   for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? 4 * g[x] : 0;
   __syncthreads();
   const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
   const uint32_t in = inst[b];
+  const uint32_t* thr = ex.thr ? ex.thr + (size_t)b * P * D : nullptr;
   for (int r = wv; r < R; r += W)
-    arpeggio_anneal<P, CPL, MODE, FULL, DS>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)r, in, k0, k1, start, out_s,
-                                            out_de,
-                                  out_st, (size_t)b * R + r, thr ? thr + (size_t)b * P * D : nullptr, gate, used);
+    arpeggio_anneal<P, CPL, MODE, FULL, DS>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)(ex.r0 + r), in, k0, k1, start,
+                                            out_s, out_de, out_st, (size_t)slot * R + r, thr, ex.gate, ex.used);
 }
 
 template <int P, int CPL, int MODE, bool FULL, int DS>
 static void launch_pcmf(dim3 grid, dim3 block, size_t lds, hipStream_t st, const int32_t* Gt, const int32_t* h,
                         const int32_t* T, int L, int q, int R, int D, const uint32_t* inst, uint32_t k0, uint32_t k1,
-                        int start, int8_t* out_s, long long* out_de, long long* out_st, const uint32_t* thr, int gate,
-                        int8_t* used) {
+                        int start, int8_t* out_s, long long* out_de, long long* out_st, const AnnealExtra& ex) {
   if (lds > 65536)
     hipFuncSetAttribute((const void*)arpeggio_fx_kernel<P, CPL, MODE, FULL, DS>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((arpeggio_fx_kernel<P, CPL, MODE, FULL, DS>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0,
-                     k1, start, out_s, out_de, out_st, thr, gate, used);
+                     k1, start, out_s, out_de, out_st, ex);
 }
 
 // FULL: L is a whole number of 4 P-pass Philox blocks, so the pass loop needs no bound test
 template <int P, int CPL, int MODE>
 static void launch_pcm(dim3 grid, dim3 block, size_t lds, hipStream_t st, const int32_t* Gt, const int32_t* h,
                        const int32_t* T, int L, int q, int R, int D, const uint32_t* inst, uint32_t k0, uint32_t k1,
-                       int start, int8_t* out_s, long long* out_de, long long* out_st, const uint32_t* thr, int gate,
-                       int8_t* used) {
+                       int start, int8_t* out_s, long long* out_de, long long* out_st, const AnnealExtra& ex) {
   // 16, 32 and 64 users (D = 32, 64, 128; two copies, first accept): D as a compile-time constant
   constexpr bool STATIC = P == 2 && MODE == 0;
   constexpr int DSA = STATIC ? (CPL == 1 ? 64 : 128) : 0, DSB = STATIC && CPL == 1 ? 32 : 0;
+  const bool full = L % (4 * P) == 0;
+#define ARP_PCMF(F, DSV) \
+  launch_pcmf<P, CPL, MODE, F, DSV>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, \
+                                    out_st, ex)
   if (STATIC && (D == DSA || D == DSB)) {
-    const bool full = L % (4 * P) == 0;
-    if (D == DSA && full)
-      launch_pcmf<P, CPL, MODE, true, DSA>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s,
-                                           out_de, out_st, thr, gate, used);
-    else if (D == DSA)
-      launch_pcmf<P, CPL, MODE, false, DSA>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s,
-                                            out_de, out_st, thr, gate, used);
-    else if (full)
-      launch_pcmf<P, CPL, MODE, true, DSB>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s,
-                                           out_de, out_st, thr, gate, used);
-    else
-      launch_pcmf<P, CPL, MODE, false, DSB>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s,
-                                            out_de, out_st, thr, gate, used);
+    if (D == DSA && full) ARP_PCMF(true, DSA);
+    else if (D == DSA) ARP_PCMF(false, DSA);
+    else if (full) ARP_PCMF(true, DSB);
+    else ARP_PCMF(false, DSB);
     return;
   }
-  if (L % (4 * P) == 0)
-    launch_pcmf<P, CPL, MODE, true, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de,
-                                       out_st, thr, gate, used);
-  else
-    launch_pcmf<P, CPL, MODE, false, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de,
-                                        out_st, thr, gate, used);
+  if (full) ARP_PCMF(true, 0);
+  else ARP_PCMF(false, 0);
+#undef ARP_PCMF
 }
 
 template <int P>
 static hipError_t launch_p(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int R, int D, int B,
                            const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode, int8_t* out_s,
-                           long long* out_de, long long* out_st, int waves, const uint32_t* thr, int gate,
-                           int8_t* used, hipStream_t st) {
+                           long long* out_de, long long* out_st, int waves, const AnnealExtra& ex, hipStream_t st) {
   const int cpl = D <= ARP_WAVE ? 1 : 2;
   const size_t lds = sizeof(int32_t) * ((size_t)D * D + ARP_WAVE * cpl);
   dim3 grid(B), block(ARP_WAVE * waves);
   if (cpl == 1) {
-    if (mode == 0) launch_pcm<P, 1, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, thr, gate, used);
-    else launch_pcm<P, 1, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, thr, gate, used);
+    if (mode == 0) launch_pcm<P, 1, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
+    else launch_pcm<P, 1, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
   } else {
-    if (mode == 0) launch_pcm<P, 2, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, thr, gate, used);
-    else launch_pcm<P, 2, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, thr, gate, used);
+    if (mode == 0) launch_pcm<P, 2, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
+    else launch_pcm<P, 2, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
   }
   return hipGetLastError();
 }
@@ -424,7 +424,7 @@ static hipError_t launch_p(const int32_t* Gt, const int32_t* h, const int32_t* T
 // The counter is cleared by arpeggio_batch_reset_kernel on the same stream before each launch.
 __global__ void arpeggio_batch_reset_kernel(unsigned* counter) { *counter = 0u; }
 
-__global__ __launch_bounds__(1024) void arpeggio_batched_kernel(ArpBatchArgs a, unsigned* __restrict__ counter) {
+__global__ __launch_bounds__(ARP_BATCHED_MAX_THREADS) void arpeggio_batched_kernel(ArpBatchArgs a, unsigned* __restrict__ counter) {
   extern __shared__ int32_t sG[];
   __shared__ int s_tile;
   const int W = blockDim.x >> 6, wv = threadIdx.x >> 6;
@@ -496,15 +496,24 @@ hipError_t launch_arpeggio_batched(const ArpBatchArgs& a, unsigned* counter, int
   return hipGetLastError();
 }
 
+hipError_t launch_arpeggio_ex(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R, int D,
+                              int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode,
+                              int8_t* out_s, long long* out_de, long long* out_st, int waves, const AnnealExtra& ex,
+                              hipStream_t st) {
+  switch (P) {
+    case 2: return launch_p<2>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, ex, st);
+    case 4: return launch_p<4>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, ex, st);
+    default: return launch_p<6>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, ex, st);
+  }
+}
+
 hipError_t launch_arpeggio_fx(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P, int R, int D,
                               int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start, int mode,
                               int8_t* out_s, long long* out_de, long long* out_st, int waves, hipStream_t st,
                               const uint32_t* thr, int gate, int8_t* used) {
-  switch (P) {
-    case 2: return launch_p<2>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, thr, gate, used, st);
-    case 4: return launch_p<4>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, thr, gate, used, st);
-    default: return launch_p<6>(Gt, h, T, L, q, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, thr, gate, used, st);
-  }
+  AnnealExtra ex{thr, used, nullptr, nullptr, gate, 0};
+  return launch_arpeggio_ex(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, start, mode, out_s, out_de, out_st, waves, ex,
+                            st);
 }
 
 }  // namespace arp

@@ -12,11 +12,12 @@
 // the best state are scalar work; fields u are per lane. A committed flip of spin j reads row j of J (= column j)
 // once, N consecutive int16 words, and updates every lane's fields.
 #include "kernels.h"
+#include "launch_bounds.h"
 
 namespace arp {
 
 template <int MPL, bool LDSJ>
-__global__ __launch_bounds__(1024) void chord_fx_kernel(const int16_t* __restrict__ Jg, const int32_t* __restrict__ hg,
+__global__ __launch_bounds__(ARP_CHORD_MAX_THREADS) void chord_fx_kernel(const int16_t* __restrict__ Jg, const int32_t* __restrict__ hg,
                                                        const int32_t* __restrict__ h0g, const int32_t* __restrict__ Tq,
                                                        const int32_t* __restrict__ Cq, int q, int S, int P, int R,
                                                        int N, uint32_t k0, uint32_t k1, int inst0, int mode,

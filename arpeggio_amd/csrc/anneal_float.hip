@@ -8,6 +8,7 @@
 // FP32 twins, so fields, energies and trajectories are reproducible bit for bit.
 // Launch shapes and register layout follow the fixed-point kernels (anneal_arpeggio.hip, anneal_chord.hip).
 #include "kernels.h"
+#include "launch_bounds.h"
 
 namespace arp {
 
@@ -16,7 +17,7 @@ __device__ __forceinline__ float uniform_r(uint32_t x, float T4) {
 }
 
 template <int P, int CPL>
-__global__ __launch_bounds__(512) void arpeggio_f32_kernel(const float* __restrict__ Gt, const float* __restrict__ h,
+__global__ __launch_bounds__(ARP_ARPEGGIO_F32_MAX_THREADS) void arpeggio_f32_kernel(const float* __restrict__ Gt, const float* __restrict__ h,
                                                            const float* __restrict__ T, int L, float q, int R, int D,
                                                            const uint32_t* __restrict__ inst, uint32_t k0, uint32_t k1,
                                                            int start, int8_t* __restrict__ out_s,
@@ -186,7 +187,7 @@ hipError_t launch_arpeggio_f32(const float* Gt, const float* h, const float* T, 
 
 // FP32 Chord: J (B, N, N) float symmetric (row j = column j), h, h0 (B, N), T, c (S) float, q float.
 template <int MPL>
-__global__ __launch_bounds__(256) void chord_f32_kernel(const float* __restrict__ Jg, const float* __restrict__ hg,
+__global__ __launch_bounds__(ARP_CHORD_F32_MAX_THREADS) void chord_f32_kernel(const float* __restrict__ Jg, const float* __restrict__ hg,
                                                         const float* __restrict__ h0g, const float* __restrict__ Tf,
                                                         const float* __restrict__ Cf, float q, int S, int P, int R,
                                                         int N, uint32_t k0, uint32_t k1, int inst0,

@@ -6,6 +6,8 @@
 
 #include "../../include/arpeggio_hip.h"
 #include "kernels.h"
+#include "launch_bounds.h"
+#include "adaptive.h"
 
 namespace arp {
 
@@ -52,15 +54,26 @@ static size_t device_max_lds() {
 
 // Anneals (waves) per workgroup of the Arpeggio kernels: a frame of D <= 64 stages 17 KiB of Gram, so four waves
 // per workgroup already fill a CU; a frame of D <= 128 stages 65 KiB (two workgroups per CU), so eight waves. The
-// kernels are compiled for workgroups of up to 512 threads (eight waves).
+// kernels are compiled for workgroups of up to ARP_ARPEGGIO_MAX_THREADS threads.
+static thread_local int g_waves_override = 0;
+
 static int arpeggio_waves(int R, int D) {
-  const int w = D > 64 ? 8 : 4;
+  const int w = g_waves_override > 0 ? g_waves_override : (D > 64 ? 8 : 4);
   return R < w ? R : w;
 }
 
+// A workgroup of `waves` wave64s must fit the kernel's compiled bound.
+static bool block_fits(int waves, int max_threads) { return waves >= 1 && ARP_WAVE * waves <= max_threads; }
+
 extern "C" const char* arp_last_error(void) { return g_err.c_str(); }
 
-extern "C" int arp_abi_version(void) { return 1; }
+extern "C" int arp_abi_version(void) { return 2; }
+
+extern "C" int arp_set_anneal_waves(int waves) {
+  if (waves < 0 || waves > 16) return fail(ARP_E_ARG, "set_anneal_waves: %d outside 0..16", waves);
+  g_waves_override = waves;
+  return ok();
+}
 
 extern "C" int arp_philox_kat(const uint32_t* in, uint32_t* out, int n, void* stream) {
   if (n < 0) return fail(ARP_E_ARG, "philox_kat: n = %d < 0", n);
@@ -83,6 +96,9 @@ extern "C" int arp_anneal_arpeggio(const int32_t* Gt, const int32_t* h, const in
   if (B == 0) return ok();
   if (!Gt || !h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio: null pointer");
   const int waves = arpeggio_waves(R, D);
+  if (!block_fits(waves, ARP_ARPEGGIO_MAX_THREADS))
+    return fail(ARP_E_ARG, "arpeggio: %d anneals per workgroup (%d threads) exceed the kernel's bound of %d threads",
+                waves, ARP_WAVE * waves, ARP_ARPEGGIO_MAX_THREADS);
   return hip_check(arp::launch_arpeggio_fx(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, start, mode, out_s,
                                            (long long*)out_de, (long long*)out_st, waves, (hipStream_t)stream),
                    "arpeggio launch");
@@ -99,8 +115,11 @@ extern "C" int arp_anneal_chord(const int16_t* J, const int32_t* h, const int32_
   if (B == 0) return ok();
   if (!J || !h || !h0 || (S > 0 && (!Tq || !Cq)) || !out_e || !out_s) return fail(ARP_E_ARG, "chord: null pointer");
   // a frame with N > 128 stages up to 128 KiB of J, one workgroup per CU: sixteen anneals per workgroup fill it
-  const int wmax = N > 128 ? 16 : 4;
+  const int wmax = g_waves_override > 0 ? g_waves_override : (N > 128 ? 16 : 4);
   const int waves = R < wmax ? R : wmax;
+  if (!block_fits(waves, ARP_CHORD_MAX_THREADS))
+    return fail(ARP_E_ARG, "chord: %d anneals per workgroup (%d threads) exceed the kernel's bound of %d threads", waves,
+                ARP_WAVE * waves, ARP_CHORD_MAX_THREADS);
   return hip_check(arp::launch_chord_fx(J, h, h0, Tq, Cq, q, S, P, R, N, B, k0, k1, inst0, mode, out_e, out_s,
                                         (unsigned long long*)stats, waves, device_max_lds(), (hipStream_t)stream),
                    "chord launch");
@@ -151,6 +170,9 @@ extern "C" int arp_anneal_arpeggio_f32(const float* Gt, const float* h, const fl
   if (B == 0) return ok();
   if (!Gt || !h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio_f32: null pointer");
   const int waves = arpeggio_waves(R, D);
+  if (!block_fits(waves, ARP_ARPEGGIO_F32_MAX_THREADS))
+    return fail(ARP_E_ARG, "arpeggio_f32: %d anneals per workgroup (%d threads) exceed the kernel's bound of %d threads",
+                waves, ARP_WAVE * waves, ARP_ARPEGGIO_F32_MAX_THREADS);
   return hip_check(arp::launch_arpeggio_f32(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, start, out_s, out_de,
                                             (long long*)out_st, out_w, waves, (hipStream_t)stream),
                    "arpeggio_f32 launch");
@@ -164,7 +186,11 @@ extern "C" int arp_anneal_chord_f32(const float* J, const float* h, const float*
   if (R < 1 || S < 0 || B < 0) return fail(ARP_E_ARG, "chord_f32: R = %d, S = %d, B = %d", R, S, B);
   if (B == 0) return ok();
   if (!J || !h || !h0 || (S > 0 && (!T || !c)) || !out_e || !out_s) return fail(ARP_E_ARG, "chord_f32: null pointer");
-  const int waves = R < 4 ? R : 4;
+  const int wmax = g_waves_override > 0 ? g_waves_override : 4;
+  const int waves = R < wmax ? R : wmax;
+  if (!block_fits(waves, ARP_CHORD_F32_MAX_THREADS))
+    return fail(ARP_E_ARG, "chord_f32: %d anneals per workgroup (%d threads) exceed the kernel's bound of %d threads",
+                waves, ARP_WAVE * waves, ARP_CHORD_F32_MAX_THREADS);
   return hip_check(arp::launch_chord_f32(J, h, h0, T, c, q, S, P, R, N, B, k0, k1, inst0, out_e, out_s, out_u,
                                          (long long*)out_fl, waves, device_max_lds(), (hipStream_t)stream),
                    "chord_f32 launch");
@@ -181,6 +207,9 @@ extern "C" int arp_anneal_arpeggio_ws(const int32_t* Gt, const int32_t* h, const
   if (!Gt || !h || !T || !inst || !thr || !out_s || !out_de || !out_st)
     return fail(ARP_E_ARG, "arpeggio_ws: null pointer");
   const int waves = arpeggio_waves(R, D);
+  if (!block_fits(waves, ARP_ARPEGGIO_MAX_THREADS))
+    return fail(ARP_E_ARG, "arpeggio_ws: %d anneals per workgroup (%d threads) exceed the kernel's bound of %d threads",
+                waves, ARP_WAVE * waves, ARP_ARPEGGIO_MAX_THREADS);
   return hip_check(arp::launch_arpeggio_fx(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, 0, 0, out_s, (long long*)out_de,
                                            (long long*)out_st, waves, (hipStream_t)stream, thr, gate ? 1 : 0,
                                            out_used),
@@ -193,8 +222,9 @@ extern "C" int arp_anneal_arpeggio_batched(const arp_batch_class* classes, int n
     return fail(ARP_E_ARG, "arpeggio_batched: n_classes = %d outside 1..%d", n_classes, ARP_MAX_CLASSES);
   if (!classes || !counter) return fail(ARP_E_ARG, "arpeggio_batched: null pointer");
   if (R < 1) return fail(ARP_E_ARG, "arpeggio_batched: R = %d", R);
-  if (waves < 1 || waves > 16 || workgroups < 1)
-    return fail(ARP_E_ARG, "arpeggio_batched: workgroups = %d, waves = %d (1..16)", workgroups, waves);
+  if (!block_fits(waves, ARP_BATCHED_MAX_THREADS) || workgroups < 1)
+    return fail(ARP_E_ARG, "arpeggio_batched: workgroups = %d, waves = %d (1..%d)", workgroups, waves,
+                ARP_BATCHED_MAX_THREADS / ARP_WAVE);
   arp::ArpBatchArgs a = {};
   a.n_classes = n_classes;
   a.R = R;
@@ -302,4 +332,56 @@ extern "C" int arp_start_thresholds(const float* z, long ld, int M, int N, float
   if (M < 0 || N < 0 || !z || !thr) return fail(ARP_E_ARG, "start_thresholds: bad arguments");
   if (M == 0 || N == 0) return ok();
   return hip_check(arp::launch_start_thr(z, ld, M, N, inv_temp, thr, (hipStream_t)stream), "start_thr launch");
+}
+
+extern "C" int arp_anneal_arpeggio_list(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P,
+                                        int R, int D, int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start,
+                                        const uint32_t* thr, int gate, int r0, const int32_t* list,
+                                        const int32_t* count, int8_t* out_s, int64_t* out_de, int64_t* out_st,
+                                        int8_t* out_used, void* stream) {
+  if (P != 2 && P != 4 && P != 6) return fail(ARP_E_ARG, "arpeggio_list: P = %d, expected 2, 4 or 6", P);
+  if (D < 1 || D > 128) return fail(ARP_E_RANGE, "arpeggio_list: D = %d outside 1..128", D);
+  if (L < 1 || R < 1 || B < 0 || r0 < 0)
+    return fail(ARP_E_ARG, "arpeggio_list: L = %d, R = %d, B = %d, r0 = %d", L, R, B, r0);
+  if (start != 0 && start != 1) return fail(ARP_E_ARG, "arpeggio_list: start = %d", start);
+  if (!list != !count) return fail(ARP_E_ARG, "arpeggio_list: list and count go together");
+  if (B == 0) return ok();
+  if (!Gt || !h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio_list: null pointer");
+  const int waves = arpeggio_waves(R, D);
+  if (!block_fits(waves, ARP_ARPEGGIO_MAX_THREADS))
+    return fail(ARP_E_ARG, "arpeggio_list: %d anneals per workgroup (%d threads) exceed the kernel's bound of %d "
+                "threads", waves, ARP_WAVE * waves, ARP_ARPEGGIO_MAX_THREADS);
+  arp::AnnealExtra ex{thr, thr ? out_used : nullptr, list, count, gate ? 1 : 0, r0};
+  return hip_check(arp::launch_arpeggio_ex(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, thr ? 0 : start, 0, out_s,
+                                           (long long*)out_de, (long long*)out_st, waves, ex, (hipStream_t)stream),
+                   "arpeggio_list launch");
+}
+
+extern "C" int arp_decode_select_list(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B,
+                                      int R, int P, int D, int Mr, int M, int delta, const int8_t* ltrue,
+                                      const int32_t* list, const int32_t* count, const int8_t* prior_l,
+                                      const float* prior_d, int8_t* lhat, int32_t* biterr, float* best_d,
+                                      void* stream) {
+  if (D < 1 || D > 128 || Mr < 1 || Mr > 256) return fail(ARP_E_RANGE, "decode_list: D = %d, Mr = %d", D, Mr);
+  if (R < 0 || R > 256) return fail(ARP_E_RANGE, "decode_list: R = %d outside 0..256", R);
+  if (M != 4 && M != 16) return fail(ARP_E_ARG, "decode_list: M = %d", M);
+  if (delta && (P < 1 || P > 6)) return fail(ARP_E_ARG, "decode_list: P = %d", P);
+  if (B < 0) return fail(ARP_E_ARG, "decode_list: B = %d < 0", B);
+  if (!list != !count) return fail(ARP_E_ARG, "decode_list: list and count go together");
+  if (!prior_l != !prior_d) return fail(ARP_E_ARG, "decode_list: prior levels and distances go together");
+  if (B == 0) return ok();
+  if (!H || !y || !xg || (R > 0 && !spins)) return fail(ARP_E_ARG, "decode_list: null input");
+  if (biterr && !ltrue) return fail(ARP_E_ARG, "decode_list: biterr needs ltrue");
+  arp::DecodeExtra ex{list, count, prior_l, prior_d, best_d};
+  return hip_check(arp::launch_decode_ex(H, y, xg, spins, B, R, P, D, Mr, M, delta, ltrue, lhat, biterr, nullptr, ex,
+                                         (hipStream_t)stream),
+                   "decode_list launch");
+}
+
+extern "C" int arp_certify(const float* best_d, const float* N0, float N0_scalar, float thr_scale, int B,
+                           int32_t* list, int32_t* count, void* stream) {
+  if (B < 0) return fail(ARP_E_ARG, "certify: B = %d < 0", B);
+  if (!list || !count || (B > 0 && !best_d)) return fail(ARP_E_ARG, "certify: null pointer");
+  return hip_check(arp::launch_certify(best_d, N0, N0_scalar, thr_scale, B, list, count, (hipStream_t)stream),
+                   "certify launch");
 }

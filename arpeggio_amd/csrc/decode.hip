@@ -7,6 +7,7 @@
 // One workgroup per frame. H is staged in LDS with row stride D+1, so the lanes of a wave (one receive row each)
 // read distinct banks while the candidate's level is a broadcast.
 #include "kernels.h"
+#include "adaptive.h"
 
 #include <cstdint>
 
@@ -27,9 +28,16 @@ __global__ __launch_bounds__(256) void decode_select_kernel(const float* __restr
                                                             const int8_t* __restrict__ spins, int R, int P, int D,
                                                             int Mr, int M, int delta, const int8_t* __restrict__ ltrue,
                                                             int8_t* __restrict__ lhat, int32_t* __restrict__ biterr,
-                                                            float* __restrict__ dist_out) {
+                                                            float* __restrict__ dist_out, DecodeExtra ex) {
   extern __shared__ float sm[];
-  const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int slot = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  int b = slot;
+  if (ex.list) {
+    if (slot >= *ex.count) return;
+    b = ex.list[slot];
+  }
+  // read before any thread of the workgroup writes best_d, which may alias prior_d
+  const float pdist = ex.prior_d ? ex.prior_d[b] : 0.f;
   const int ld = D + 1, C = R + 1;
   float* Hs = sm;                 // Mr x (D+1)
   float* ys = Hs + Mr * ld;       // Mr
@@ -49,7 +57,7 @@ __global__ __launch_bounds__(256) void decode_select_kernel(const float* __restr
     if (r == R) {
       l = (float)xg[(size_t)b * D + j];
     } else {
-      const int8_t* s = spins + ((size_t)b * R + r) * N;
+      const int8_t* s = spins + ((size_t)slot * R + r) * N;
       if (delta) {
         int d = 0;
         for (int p = 0; p < P; ++p) d += s[p * D + j];
@@ -83,10 +91,12 @@ __global__ __launch_bounds__(256) void decode_select_kernel(const float* __restr
       best = c;
     }
   if (dist_out)
-    for (int c = tid; c < C; c += nt) dist_out[(size_t)b * C + c] = dv[c];
+    for (int c = tid; c < C; c += nt) dist_out[(size_t)slot * C + c] = dv[c];
+  const bool keep = ex.prior_d && !(bd < pdist);   // the prior candidate wins ties
+  if (ex.best_d && tid == 0) ex.best_d[b] = keep ? pdist : bd;
   int errs = 0;
   for (int j = tid; j < D; j += nt) {
-    const int l = (int)Lc[best * D + j];
+    const int l = keep ? (int)ex.prior_l[(size_t)b * D + j] : (int)Lc[best * D + j];
     if (lhat) lhat[(size_t)b * D + j] = (int8_t)l;
     if (ltrue) errs += __builtin_popcount(gray_bits(l, M) ^ gray_bits(ltrue[(size_t)b * D + j], M));
   }
@@ -113,9 +123,17 @@ __global__ __launch_bounds__(64) void decode_w64_kernel(const float* __restrict_
                                                         int R, int P, int D, int Mr, int M, int delta, int vec4,
                                                         int vecs,
                                                         const int8_t* __restrict__ ltrue, int8_t* __restrict__ lhat,
-                                                        int32_t* __restrict__ biterr, float* __restrict__ dist_out) {
+                                                        int32_t* __restrict__ biterr, float* __restrict__ dist_out,
+                                                        DecodeExtra ex) {
   extern __shared__ float sm[];
-  const int b = blockIdx.x, lane = threadIdx.x;
+  const int slot = blockIdx.x, lane = threadIdx.x;
+  int b = slot;
+  if (ex.list) {
+    if (slot >= *ex.count) return;
+    b = ex.list[slot];
+  }
+  // read before any thread of the workgroup writes best_d, which may alias prior_d
+  const float pdist = ex.prior_d ? ex.prior_d[b] : 0.f;
   const int C = R + 1;
   float* Lc = sm;  // C x DP candidate levels (columns beyond D are never read)
   const float* Hb = H + (size_t)b * Mr * D;
@@ -153,7 +171,7 @@ __global__ __launch_bounds__(64) void decode_w64_kernel(const float* __restrict_
 #pragma unroll
         for (int p = 0; p < 8; ++p)
           w[t][p] = (live && r < R && p < planes)
-                        ? *reinterpret_cast<const int*>(spins + ((size_t)b * R + r) * N + (size_t)p * D + 4 * q)
+                        ? *reinterpret_cast<const int*>(spins + ((size_t)slot * R + r) * N + (size_t)p * D + 4 * q)
                         : 0;
       }
 #pragma unroll
@@ -190,7 +208,7 @@ __global__ __launch_bounds__(64) void decode_w64_kernel(const float* __restrict_
         if (r == R) {
           l = (float)x0;
         } else {
-          const int8_t* s = spins + ((size_t)b * R + r) * N;
+          const int8_t* s = spins + ((size_t)slot * R + r) * N;
           if (delta) {
             int d = 0;
             for (int p = 0; p < P; ++p) d += s[p * D + lane];
@@ -219,15 +237,17 @@ __global__ __launch_bounds__(64) void decode_w64_kernel(const float* __restrict_
     const float e = yi - sacc;
     float part = row ? e * e : 0.f;
     for (int o = 32; o; o >>= 1) part += __shfl_xor(part, o);
-    if (dist_out && lane == 0) dist_out[(size_t)b * C + r] = part;
+    if (dist_out && lane == 0) dist_out[(size_t)slot * C + r] = part;
     if (r == 0 || part < bd) {
       bd = part;
       best = r;
     }
   }
+  const bool keep = ex.prior_d && !(bd < pdist);   // the prior candidate wins ties
+  if (ex.best_d && lane == 0) ex.best_d[b] = keep ? pdist : bd;
   int errs = 0;
   if (col) {
-    const int l = (int)Lc[best * DP + lane];
+    const int l = keep ? (int)ex.prior_l[(size_t)b * D + lane] : (int)Lc[best * DP + lane];
     if (lhat) lhat[(size_t)b * D + lane] = (int8_t)l;
     if (ltrue) errs = __builtin_popcount(gray_bits(l, M) ^ gray_bits(ltrue[(size_t)b * D + lane], M));
   }
@@ -245,10 +265,18 @@ __global__ __launch_bounds__(128) void decode_w128_kernel(const float* __restric
                                                           const int8_t* __restrict__ spins, int R, int P, int D,
                                                           int Mr, int M, int delta, int vec4,
                                                           const int8_t* __restrict__ ltrue, int8_t* __restrict__ lhat,
-                                                          int32_t* __restrict__ biterr, float* __restrict__ dist_out) {
+                                                          int32_t* __restrict__ biterr, float* __restrict__ dist_out,
+                                                          DecodeExtra ex) {
   constexpr int DP = 128;
   extern __shared__ float sm[];
-  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int slot = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  int b = slot;
+  if (ex.list) {
+    if (slot >= *ex.count) return;
+    b = ex.list[slot];
+  }
+  // read before any thread of the workgroup writes best_d, which may alias prior_d
+  const float pdist = ex.prior_d ? ex.prior_d[b] : 0.f;
   const int C = R + 1;
   float* Lc = sm;              // C x DP candidate levels
   float* dsum = Lc + C * DP;   // C x 2 per-wave partial distances
@@ -278,7 +306,7 @@ __global__ __launch_bounds__(128) void decode_w128_kernel(const float* __restric
       if (r == R) {
         l = (float)x0;
       } else {
-        const int8_t* s = spins + ((size_t)b * R + r) * N;
+        const int8_t* s = spins + ((size_t)slot * R + r) * N;
         if (delta) {
           int d = 0;
           for (int p = 0; p < P; ++p) d += s[p * D + t];
@@ -317,10 +345,12 @@ __global__ __launch_bounds__(128) void decode_w128_kernel(const float* __restric
     }
   }
   if (dist_out)
-    for (int r = t; r < C; r += 128) dist_out[(size_t)b * C + r] = dsum[2 * r] + dsum[2 * r + 1];
+    for (int r = t; r < C; r += 128) dist_out[(size_t)slot * C + r] = dsum[2 * r] + dsum[2 * r + 1];
+  const bool keep = ex.prior_d && !(bd < pdist);   // the prior candidate wins ties
+  if (ex.best_d && t == 0) ex.best_d[b] = keep ? pdist : bd;
   int errs = 0;
   if (col) {
-    const int l = (int)Lc[best * DP + t];
+    const int l = keep ? (int)ex.prior_l[(size_t)b * D + t] : (int)Lc[best * DP + t];
     if (lhat) lhat[(size_t)b * D + t] = (int8_t)l;
     if (ltrue) errs = __builtin_popcount(gray_bits(l, M) ^ gray_bits(ltrue[(size_t)b * D + t], M));
   }
@@ -340,6 +370,13 @@ size_t decode_lds_bytes(int R, int D, int Mr) {
 hipError_t launch_decode_select(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R,
                                 int P, int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat,
                                 int32_t* biterr, float* dist_out, hipStream_t st) {
+  return launch_decode_ex(H, y, xg, spins, B, R, P, D, Mr, M, delta, ltrue, lhat, biterr, dist_out,
+                          DecodeExtra{nullptr, nullptr, nullptr, nullptr, nullptr}, st);
+}
+
+hipError_t launch_decode_ex(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R, int P,
+                            int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat, int32_t* biterr,
+                            float* dist_out, const DecodeExtra& ex, hipStream_t st) {
   if (D <= 64 && Mr <= 64) {
     const int DP = (D + 15) & ~15;
     const size_t l64 = sizeof(float) * (size_t)(R + 1) * DP;
@@ -349,7 +386,7 @@ hipError_t launch_decode_select(const float* H, const float* y, const int8_t* xg
                      ((reinterpret_cast<uintptr_t>(xg) & 3) == 0);
 #define ARP_DECODE_W64(DPv)                                                                                        \
   hipLaunchKernelGGL((decode_w64_kernel<DPv>), dim3(B), dim3(64), l64, st, H, y, xg, spins, R, P, D, Mr, M, delta,   \
-                     vec4, vecs, ltrue, lhat, biterr, dist_out)
+                     vec4, vecs, ltrue, lhat, biterr, dist_out, ex)
     if (DP == 16) ARP_DECODE_W64(16);
     else if (DP == 32) ARP_DECODE_W64(32);
     else if (DP == 48) ARP_DECODE_W64(48);
@@ -363,14 +400,14 @@ hipError_t launch_decode_select(const float* H, const float* y, const int8_t* xg
       hipFuncSetAttribute((const void*)decode_w128_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l128);
     const int vec4 = (D % 4 == 0) && ((reinterpret_cast<uintptr_t>(H) & 15) == 0);
     hipLaunchKernelGGL(decode_w128_kernel, dim3(B), dim3(128), l128, st, H, y, xg, spins, R, P, D, Mr, M, delta, vec4,
-                       ltrue, lhat, biterr, dist_out);
+                       ltrue, lhat, biterr, dist_out, ex);
     return hipGetLastError();
   }
   const size_t lds = decode_lds_bytes(R, D, Mr);
   if (lds > 65536)
     hipFuncSetAttribute((const void*)decode_select_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(decode_select_kernel, dim3(B), dim3(256), lds, st, H, y, xg, spins, R, P, D, Mr, M, delta, ltrue,
-                     lhat, biterr, dist_out);
+                     lhat, biterr, dist_out, ex);
   return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@ levels -> select) with the fixed-point Arpeggio solver of arpeggio/sim/local_cod
 import torch
 
 from . import _lib
+from .arpeggiofx import schedule_check
 from .chordfx import QSCALE, geom, to_fixed
 
 MASK = 0xFFFFFFFF
@@ -97,6 +98,7 @@ class DetectPipeline:
         if policy == 'arpeggio':
             assert cycles % self.P == 0, 'arpeggio cycles = passes, a multiple of P'
             T, qi = arpeggio_schedule(cycles // self.P, T0, T1, q, self.K, Q)
+            schedule_check(T, qi, self.P, D, Q)
             self.T = torch.tensor(T, dtype=torch.int32, device=dev)
             self.q = qi
             self.Gi = torch.empty(B, D, D, dtype=torch.int32, device=dev)
@@ -126,9 +128,27 @@ class DetectPipeline:
             raise ValueError('policy must be arpeggio or chord')
         self.Q = float(Q)
 
+    def _check_inputs(self, H, y, N0, ltrue):
+        """The kernels read the buffers by raw pointer: refuse anything but the pipeline's own shapes, float32 / int8,
+        contiguous, on the GPU."""
+        B, Mr, D = self.B, self.Mr, self.D
+        for name, t, shape, dt in (('H', H, (B, Mr, D), torch.float32), ('y', y, (B, Mr), torch.float32),
+                                   ('ltrue', ltrue, (B, D), torch.int8)):
+            if t is None:
+                continue
+            if tuple(t.shape) != shape or t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+                raise ValueError('DetectPipeline.run: %s must be a contiguous %s GPU tensor of shape %s, got %s %s%s'
+                                 % (name, dt, shape, t.dtype, tuple(t.shape), '' if t.is_cuda else ' on the CPU'))
+        if torch.is_tensor(N0) and N0.numel() > 1:
+            if N0.numel() != B or N0.dtype != torch.float32 or not N0.is_cuda or not N0.is_contiguous():
+                raise ValueError('DetectPipeline.run: per-frame N0 must be a contiguous float32 GPU tensor of %d '
+                                 'values' % B)
+
     def run(self, H, y, N0, ltrue=None, stream=None, events=None):
-        """Launches front end, anneal and decode on one stream; returns (lhat, biterr or None) device tensors.
+        """Launches front end, anneal and decode on one stream (default: torch's current stream; the MLP of the learned
+        start runs on the same stream); returns (lhat, biterr or None) device tensors.
         events: optional (start, end) torch.cuda.Event pair recorded on the same stream around the anneal launch."""
+        self._check_inputs(H, y, N0, ltrue)
         L, s = self.lib, _lib.stream_ptr(stream)
         ts = stream if stream is not None else torch.cuda.current_stream()
 
@@ -142,7 +162,7 @@ class DetectPipeline:
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
                                        P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None,
                                        None, None, P_(self.feat), self.feat.shape[1], s), 'front_end')
-            self.model.thresholds(self.feat, self.inv_temp, out=self.thr)
+            self.model.thresholds(self.feat, self.inv_temp, out=self.thr, stream=stream)
             mark(0)
             _lib.check(L.arp_anneal_arpeggio_ws(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D,
                                                 B, P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
@@ -186,3 +206,113 @@ def detect(H, y, sigma2, policy='arpeggio', copies=16, cycles=64, seed=0, M=16, 
     lt = ltrue.to(device='cuda', dtype=torch.int8).contiguous() if ltrue is not None else None
     lhat, be = pipe.run(H, y, N0, lt)
     return (lhat, be) if ltrue is not None else lhat
+
+
+class AdaptivePipeline:
+    """Two-stage detection with an early-stop certificate (no reference counterpart for the two-stage form).
+
+    Stage 1 anneals every frame briefly (R1 anneals x L1 passes, learned or random start) and decodes it. A frame whose
+    chosen candidate already explains y within the noise, ||y - H x||^2 <= N0 (Nr + c sqrt(Nr)), is final: the
+    reference's first-pass stop rule (arpeggio/sim/a56an.py:58-67). Stage 2 anneals only the open frames, from a
+    device-side list, with R2 more anneals (Philox streams R1 .. R1 + R2 - 1) of L2 passes, and its best candidate
+    replaces stage 1's only where it is strictly closer. Every launch is stream-ordered with no host synchronisation,
+    so run() can be captured in a HIP graph.
+
+    start 'learned' needs model (a learned.WarmStartMLP for this D); the MLP runs once for both stages."""
+
+    def __init__(self, B, Mr, D, M, R1=4, L1=16, R2=16, L2=64, c=3.0, start='learned', T0=0.1, T1=0.1, q=0.0,
+                 key=0, inst0=0, Q=QSCALE, model=None, gate=True, inv_temp=None):
+        self.B, self.Mr, self.D, self.M = B, Mr, D, M
+        self.R1, self.L1, self.R2, self.L2, self.c = R1, L1, R2, L2, float(c)
+        self.P, self.K, self.key, self.start, self.Q = 2, 1, key, start, float(Q)
+        self.lib = _lib.load()
+        dev = 'cuda'
+        N = self.P * D
+        sched = []
+        for L in (L1, L2):
+            assert L % self.P == 0, 'passes must be a multiple of P'
+            T, qi = arpeggio_schedule(L // self.P, T0, T1, q, self.K, Q)
+            schedule_check(T, qi, self.P, D, Q)
+            sched.append(torch.tensor(T, dtype=torch.int32, device=dev))
+        self.T1t, self.T2t, self.q = sched[0], sched[1], qi
+        self.xg = torch.empty(B, D, dtype=torch.int8, device=dev)
+        self.Gi = torch.empty(B, D, D, dtype=torch.int32, device=dev)
+        self.hi = torch.empty(B, D, dtype=torch.int32, device=dev)
+        self.scale = torch.empty(B, dtype=torch.float32, device=dev)
+        self.inst = ((torch.arange(B, dtype=torch.int64) + inst0) & MASK).to(torch.int32).to(dev)
+        self.spins1 = torch.empty(B, R1, N, dtype=torch.int8, device=dev)
+        self.de1 = torch.empty(B, R1, dtype=torch.int64, device=dev)
+        self.st1 = torch.empty(B, R1, 2, dtype=torch.int64, device=dev)
+        self.spins2 = torch.empty(B, R2, N, dtype=torch.int8, device=dev)
+        self.de2 = torch.empty(B, R2, dtype=torch.int64, device=dev)
+        self.st2 = torch.empty(B, R2, 2, dtype=torch.int64, device=dev)
+        self.d1 = torch.empty(B, dtype=torch.float32, device=dev)
+        self.d2 = torch.empty(B, dtype=torch.float32, device=dev)
+        self.lhat = torch.empty(B, D, dtype=torch.int8, device=dev)
+        self.biterr = torch.empty(B, dtype=torch.int32, device=dev)
+        self.list = torch.empty(B, dtype=torch.int32, device=dev)
+        self.count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.thr = None
+        if start == 'learned':
+            from .learned import feat_ld
+            assert model is not None and model.D == D and model.P == self.P, 'learned start needs a matching model'
+            self.model, self.gate = model, gate
+            self.inv_temp = model.inv_temp if inv_temp is None else inv_temp
+            self.feat = torch.empty(B, feat_ld(D), dtype=torch.bfloat16, device=dev)
+            self.thr = torch.empty(B, N, dtype=torch.int32, device=dev)
+            self.used1 = torch.empty(B, R1, dtype=torch.int8, device=dev)
+            self.used2 = torch.empty(B, R2, dtype=torch.int8, device=dev)
+        elif start != 'rand':
+            raise ValueError("start must be 'learned' or 'rand'")
+        nr = Mr // 2
+        self.thr_scale = nr + self.c * nr ** 0.5
+
+    _check_inputs = DetectPipeline._check_inputs
+
+    def run(self, H, y, N0, ltrue=None, stream=None, events=None):
+        """Both stages on one stream; returns (lhat, biterr or None). events: optional (start, end) pair recorded around
+        the two anneal stages (certificate included)."""
+        self._check_inputs(H, y, N0, ltrue)
+        L, s = self.lib, _lib.stream_ptr(stream)
+        ts = stream if stream is not None else torch.cuda.current_stream()
+        B, Mr, D, M, P = self.B, self.Mr, self.D, self.M, self.P
+        n0t, n0s = _n0_arg(N0, B)
+        P_ = _lib.ptr
+        learned = self.thr is not None
+        _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg), P_(self.Gi),
+                                   P_(self.hi), P_(self.scale), None, None, None, None, None, None, None, None,
+                                   P_(self.feat) if learned else None, self.feat.shape[1] if learned else 0, s),
+                   'front_end')
+        if learned:
+            self.model.thresholds(self.feat, self.inv_temp, out=self.thr, stream=stream)
+        if events is not None:
+            events[0].record(ts)
+        k0, k1 = self.key & MASK, (self.key >> 32) & MASK
+        gate = 1 if learned and self.gate else 0
+        _lib.check(L.arp_anneal_arpeggio_list(P_(self.Gi), P_(self.hi), P_(self.T1t), self.T1t.numel(), self.q, P,
+                                              self.R1, D, B, P_(self.inst), k0, k1, 0, P_(self.thr), gate, 0, None,
+                                              None, P_(self.spins1), P_(self.de1), P_(self.st1),
+                                              P_(self.used1) if learned else None, s), 'anneal stage 1')
+        lt = P_(ltrue)
+        _lib.check(L.arp_decode_select_list(P_(H), P_(y), P_(self.xg), P_(self.spins1), B, self.R1, P, D, Mr, M, 1,
+                                            lt, None, None, None, None, P_(self.lhat),
+                                            P_(self.biterr) if ltrue is not None else None, P_(self.d1), s),
+                   'decode stage 1')
+        _lib.check(L.arp_certify(P_(self.d1), P_(n0t), n0s, self.thr_scale, B, P_(self.list), P_(self.count), s),
+                   'certify')
+        _lib.check(L.arp_anneal_arpeggio_list(P_(self.Gi), P_(self.hi), P_(self.T2t), self.T2t.numel(), self.q, P,
+                                              self.R2, D, B, P_(self.inst), k0, k1, 0, P_(self.thr), gate, self.R1,
+                                              P_(self.list), P_(self.count), P_(self.spins2), P_(self.de2),
+                                              P_(self.st2), P_(self.used2) if learned else None, s), 'anneal stage 2')
+        if events is not None:
+            events[1].record(ts)
+        _lib.check(L.arp_decode_select_list(P_(H), P_(y), P_(self.xg), P_(self.spins2), B, self.R2, P, D, Mr, M, 1,
+                                            lt, P_(self.list), P_(self.count), P_(self.lhat), P_(self.d1),
+                                            P_(self.lhat), P_(self.biterr) if ltrue is not None else None,
+                                            P_(self.d2), s), 'decode stage 2')
+        return self.lhat, (self.biterr if ltrue is not None else None)
+
+    def open_frames(self):
+        """Host copy of the frame indices stage 2 annealed (sorted)."""
+        n = int(self.count.item())
+        return torch.sort(self.list[:n]).values.cpu()

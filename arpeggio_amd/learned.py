@@ -75,24 +75,26 @@ class WarmStartMLP:
         return m
 
     # ---------------------------------------------------------------- kernels
-    def _gemm(self, A, sam, sak, B, sbk, sbn, M, N, K, bias=None, act=0, Cf=None, Cpre=None, Cb=None, ldc=None):
+    def _gemm(self, A, sam, sak, B, sbk, sbn, M, N, K, bias=None, act=0, Cf=None, Cpre=None, Cb=None, ldc=None,
+              stream=None):
         p = _lib.ptr
         _lib.check(self.lib.arp_mlp_gemm(p(A), sam, sak, p(B), sbk, sbn, M, N, K, p(bias), act, p(Cf), p(Cpre), p(Cb),
-                                         ldc if ldc is not None else N, _lib.stream_ptr()), 'mlp_gemm')
+                                         ldc if ldc is not None else N, _lib.stream_ptr(stream)), 'mlp_gemm')
 
-    def forward(self, X, save=False):
-        """X (M, F) bf16 features -> logits (M, N) fp32 (and the activations backward needs if save)."""
+    def forward(self, X, save=False, stream=None):
+        """X (M, F) bf16 features -> logits (M, N) fp32 (and the activations backward needs if save). The GEMMs run on
+        `stream` (default: torch's current stream); the outputs are allocated on the current stream's pool."""
         M, H, N, F = X.shape[0], self.H, self.N, self.F
         dev = X.device
         P_ = self.params
         A1 = torch.empty(M, H, device=dev) if save else None
         H1 = torch.empty(M, H, dtype=torch.bfloat16, device=dev)
-        self._gemm(X, F, 1, self.wb['W1'], H, 1, M, H, F, P_['b1'], 1, None, A1, H1)
+        self._gemm(X, F, 1, self.wb['W1'], H, 1, M, H, F, P_['b1'], 1, None, A1, H1, stream=stream)
         A2 = torch.empty(M, H, device=dev) if save else None
         H2 = torch.empty(M, H, dtype=torch.bfloat16, device=dev)
-        self._gemm(H1, H, 1, self.wb['W2'], H, 1, M, H, H, P_['b2'], 1, None, A2, H2)
+        self._gemm(H1, H, 1, self.wb['W2'], H, 1, M, H, H, P_['b2'], 1, None, A2, H2, stream=stream)
         Z = torch.empty(M, N, device=dev)
-        self._gemm(H2, H, 1, self.wb['W3'], N, 1, M, N, H, P_['b3'], 0, Z)
+        self._gemm(H2, H, 1, self.wb['W3'], N, 1, M, N, H, P_['b3'], 0, Z, stream=stream)
         if save:
             return Z, dict(X=X, A1=A1, H1=H1, A2=A2, H2=H2)
         return Z
@@ -151,9 +153,10 @@ class WarmStartMLP:
             self._wt_step, self._wt_src = self.step_count, self.wb
         return self._wt
 
-    def thresholds(self, X, inv_temp=1.0, out=None, fused=True):
+    def thresholds(self, X, inv_temp=1.0, out=None, fused=True, stream=None):
         """Per-spin start thresholds (M, N) uint32 (as int32 storage) from the features: one fused kernel
-        (arp_mlp_start_fused), or the three GEMMs and the threshold kernel (fused=False); both give the same bits."""
+        (arp_mlp_start_fused), or the three GEMMs and the threshold kernel (fused=False); both give the same bits.
+        The kernels run on `stream` (default: torch's current stream), ordered after the work already queued there."""
         M, N = X.shape[0], self.N
         thr = out if out is not None else torch.empty(M, N, dtype=torch.int32, device=X.device)
         if fused and self.H == 256 and N % 64 == 0 and N <= 256 and self.F <= 504:
@@ -162,11 +165,13 @@ class WarmStartMLP:
             p = _lib.ptr
             _lib.check(self.lib.arp_mlp_start_fused(p(X), M, self.F, p(W['W1']), p(P_['b1']), p(W['W2']), p(P_['b2']),
                                                     p(W['W3']), p(P_['b3']), self.H, N, float(inv_temp), p(thr),
-                                                    _lib.stream_ptr()), 'mlp_start_fused')
+                                                    _lib.stream_ptr(stream)), 'mlp_start_fused')
             return thr
-        Z = self.forward(X)
+        Z = self.forward(X, stream=stream)
+        if stream is not None:
+            Z.record_stream(stream)
         _lib.check(self.lib.arp_start_thresholds(_lib.ptr(Z), N, M, N, float(inv_temp), _lib.ptr(thr),
-                                                 _lib.stream_ptr()), 'start_thresholds')
+                                                 _lib.stream_ptr(stream)), 'start_thresholds')
         return thr
 
 

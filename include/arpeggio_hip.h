@@ -31,6 +31,12 @@ const char *arp_last_error(void);
 /* ABI version of this library (incremented when a signature changes). */
 int arp_abi_version(void);
 
+/* Anneals (wave64s) per workgroup of the annealing entry points called afterwards on this thread: 0 restores the
+ * library's choice (4 for D <= 64 and 8 beyond for the first-accept kernels, 4 or 16 for Chord); a choice whose
+ * workgroup exceeds a kernel's compiled bound makes that entry point return ARP_E_ARG before anything is launched.
+ * A tuning knob with no reference counterpart (the reference launches 128-thread blocks, arpeggiofx.py:39-42). */
+int arp_set_anneal_waves(int waves);
+
 /* Philox4x32-10 known-answer kernel: in (n, 6) uint32 rows (counter[4], key[2]) -> out (n, 4).
  * Replaces the reference's device KAT `philox_kat` (arpeggio/mimo/chord_kernel.cu:26-32), driven by
  * Kernel.philox_kat (arpeggio/mimo/chordfx.py:180-187). */
@@ -44,7 +50,9 @@ int arp_philox_kat(const uint32_t *in, uint32_t *out, int n, void *stream);
  *   inst  (B) uint32 Philox instance words    k0, k1 = low, high 32 bits of the 64-bit key
  *   start 0 'rand', 1 'bal'
  *   out_s (B, R, P*D) int8 best spins, copy-major (spin p*D + j)   out_de (B, R) int64 best energy change (2 E0, Q)
- *   out_st (B, R, 2) int64 (flips, tests) */
+ *   out_st (B, R, 2) int64 (flips, tests)
+ * Fields and decision bounds are 32-bit here where the reference compares in 64 bits: 0 <= T < 2^29, and
+ * max_i(|h_i| + 2 P sum_j |G_ij|) + 2 max|G_jj| + |q| + 2 max T < 2^31 (the front end's |G|, |h| <= Q keep this). */
 int arp_anneal_arpeggio(const int32_t *Gt, const int32_t *h, const int32_t *T, int L, int q, int P, int R, int D,
                         int B, const uint32_t *inst, uint32_t k0, uint32_t k1, int start, int mode, int8_t *out_s,
                         int64_t *out_de, int64_t *out_st, void *stream);
@@ -59,7 +67,8 @@ int arp_anneal_chord(const int16_t *J, const int32_t *h, const int32_t *h0, cons
                      int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1, int inst0, int mode,
                      int32_t *out_e, int8_t *out_s, uint64_t *stats, void *stream);
 
-/* Fused front end, one workgroup per frame: LMMSE hard decision (Cholesky of H^T H + (N0/2)/E[l^2] I),
+/* Fused front end, one workgroup per frame: LMMSE hard decision (in-register symmetric-sweep inversion of
+ * H^T H + (N0/2)/E[l^2] I; one wave per frame for D <= 64, two for D <= 128),
  * delta-Ising build around it (dDI, 2K copies) on the matrix cores, and the fixed-point inputs of the annealers.
  * Replaces, on the kernel path, mimo.mmse / mmse_hard / _mu / quantize_levels (arpeggio/mimo/mimo.py:86-109, 176-182),
  * mimo.ising_delta_at (mimo.py:506-523), the J/h rounding of chordfx.to_fixed (arpeggio/mimo/chordfx.py:60-76) and
@@ -108,7 +117,8 @@ int arp_anneal_chord_f32(const float *J, const float *h, const float *h0, const 
  *   thr (B, P*D) uint32 per-spin start thresholds: spin +1 iff x + thr >= 2^32, x the random start's Philox word
  *       (thr = 2^31 reproduces the random start exactly)
  *   gate 1: per anneal, keep the learned start only if its Ising energy is not above the random start's
- *   out_used (B, R) int8 or NULL: 1 where the learned start was kept. */
+ *   out_used (B, R) int8 or NULL: 1 where the learned start was kept.
+ * Same integer ranges as arp_anneal_arpeggio. */
 int arp_anneal_arpeggio_ws(const int32_t *Gt, const int32_t *h, const int32_t *T, int L, int q, int P, int R, int D,
                            int B, const uint32_t *inst, uint32_t k0, uint32_t k1, const uint32_t *thr, int gate,
                            int8_t *out_s, int64_t *out_de, int64_t *out_st, int8_t *out_used, void *stream);
@@ -132,9 +142,40 @@ typedef struct {
  * the largest first). Each anneal equals arp_anneal_arpeggio's on the same class buffers, stream for stream. The
  * reference launches one fixed shape per call (arpeggio_fx.cu:35-37); packing several shapes into one launch has no
  * counterpart there. counter: one device uint32 of scratch (cleared on the stream before the launch).
- * n_classes <= 8, D <= 128 per class, waves 1..16. */
+ * n_classes <= 8, D <= 128 per class, waves 1..16. Same integer ranges as arp_anneal_arpeggio per class. */
 int arp_anneal_arpeggio_batched(const arp_batch_class *classes, int n_classes, int R, uint32_t k0, uint32_t k1,
                                 uint32_t *counter, int workgroups, int waves, void *stream);
+
+/* ---- Adaptive (early-stop) detection: a short anneal for every frame, a certificate, and a longer anneal for the
+ * frames the certificate leaves open (no reference counterpart for the two-stage form; the stopping rule is the
+ * reference's first-pass stop t = ||y - H x||^2 / N0 <= Nr + c sqrt(Nr), arpeggio/sim/a56an.py:58-67). */
+
+/* First-accept anneal over a frame list: workgroup slot i < *count anneals frame list[i] (arrays Gt, h, inst, thr
+ * indexed by frame) with Philox streams r0 .. r0 + R - 1 and writes out_s (slots, R, P D), out_de (slots, R),
+ * out_st (slots, R, 2), out_used (slots, R) at slot i; slots at or past *count do nothing. B = number of slots
+ * launched (the list's capacity). list = count = NULL: every frame b < B in its own slot (then r0 = 0 equals
+ * arp_anneal_arpeggio / arp_anneal_arpeggio_ws). thr = NULL: the start `start` (0 rand, 1 bal); else the learned start
+ * with the energy gate `gate` (as arp_anneal_arpeggio_ws). Anneal r of a frame is the same stream in every launch, so
+ * r0 = R1 continues a frame's R1 anneals of an earlier launch with new ones. Mode: first accept. */
+int arp_anneal_arpeggio_list(const int32_t *Gt, const int32_t *h, const int32_t *T, int L, int q, int P, int R, int D,
+                             int B, const uint32_t *inst, uint32_t k0, uint32_t k1, int start, const uint32_t *thr,
+                             int gate, int r0, const int32_t *list, const int32_t *count, int8_t *out_s,
+                             int64_t *out_de, int64_t *out_st, int8_t *out_used, void *stream);
+
+/* Decode and select over a frame list (list/count as arp_anneal_arpeggio_list; NULL: every frame), with an optional
+ * prior candidate per frame (prior_l (B, D) int8 levels, prior_d (B) float32 distance) that the best new candidate
+ * must beat strictly. Writes lhat, biterr and best_d (the chosen candidate's ||y - H x||^2; each may be NULL) at the
+ * frame's index. Spins are read from the slot. Replaces the same reference functions as arp_decode_select. */
+int arp_decode_select_list(const float *H, const float *y, const int8_t *xg, const int8_t *spins, int B, int R, int P,
+                           int D, int Mr, int M, int delta, const int8_t *ltrue, const int32_t *list,
+                           const int32_t *count, const int8_t *prior_l, const float *prior_d, int8_t *lhat,
+                           int32_t *biterr, float *best_d, void *stream);
+
+/* Certificate: frames b < B with best_d[b] > N0_b * thr_scale (N0_b = N0[b], or N0_scalar if N0 is NULL; thr_scale =
+ * Nr + c sqrt(Nr)) are written to list (order within a wave by frame, across waves unspecified); *count = their
+ * number (cleared on the stream first). Graph-capturable: no memset node. */
+int arp_certify(const float *best_d, const float *N0, float N0_scalar, float thr_scale, int B, int32_t *list,
+                int32_t *count, void *stream);
 
 /* Learned-start MLP kernels (no reference counterpart). bf16 tensors are passed as their uint16 bit patterns.
  * arp_mlp_gemm: C(m, n) = sum_k A(m, k) B(k, n) (+ bias[n]), act 0 none / 1 GELU (tanh form), on v_mfma_f32_16x16x32_bf16

@@ -51,3 +51,31 @@ def test_batched_argument_errors_without_gpu():
     assert L.arp_anneal_arpeggio_batched(ct.cast(arr, ct.c_void_p), 0, 4, 0, 0, ct.c_void_p(16), 4, 4, None) == -1
     arr[0] = _lib.BatchClass(None, None, None, None, None, None, None, 64, 0, 8, 0, 0, 1)
     assert L.arp_anneal_arpeggio_batched(ct.cast(arr, ct.c_void_p), 1, 4, 0, 0, ct.c_void_p(16), 4, 4, None) == 0
+
+
+def test_oversized_workgroup_refused_without_gpu():
+    """A launch shape above a kernel's compiled workgroup bound is an argument error on the host, before any HIP call:
+    16 anneals per workgroup (1,024 threads) against the first-accept kernels' 512 and the FP32 Chord's 256. The
+    pointers are host placeholders that no kernel ever sees."""
+    import ctypes as ct
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    buf = ct.create_string_buffer(64)
+    p = ct.cast(buf, ct.c_void_p)
+    try:
+        assert L.arp_set_anneal_waves(17) == -1
+        assert L.arp_set_anneal_waves(16) == 0
+        rc = L.arp_anneal_arpeggio(p, p, p, 8, 0, 2, 16, 128, 1, p, 0, 0, 0, 0, p, p, p, None)
+        assert rc == -1 and b'exceed the kernel' in L.arp_last_error(), L.arp_last_error()
+        rc = L.arp_anneal_arpeggio_ws(p, p, p, 8, 0, 2, 16, 64, 1, p, 0, 0, p, 1, p, p, p, None, None)
+        assert rc == -1 and b'exceed the kernel' in L.arp_last_error()
+        rc = L.arp_anneal_arpeggio_f32(p, p, p, 8, 0.0, 2, 16, 64, 1, p, 0, 0, 0, p, p, p, None, None)
+        assert rc == -1 and b'exceed the kernel' in L.arp_last_error()
+        rc = L.arp_anneal_chord_f32(p, p, p, p, p, 0.0, 8, 2, 16, 64, 1, 0, 0, 0, p, p, None, None, None)
+        assert rc == -1 and b'exceed the kernel' in L.arp_last_error()
+        arr = (_lib.BatchClass * 1)()
+        arr[0] = _lib.BatchClass(p.value, p.value, p.value, p.value, p.value, p.value, p.value, 64, 4, 8, 0, 0, 1)
+        rc = L.arp_anneal_arpeggio_batched(ct.cast(arr, ct.c_void_p), 1, 4, 0, 0, p, 4, 17, None)
+        assert rc == -1 and b'waves = 17' in L.arp_last_error()
+    finally:
+        assert L.arp_set_anneal_waves(0) == 0

@@ -1,0 +1,122 @@
+"""GPU parity of the adaptive (two-stage, early-stop) detector: every stage against the oracle composed the same way.
+
+Stage 1 anneals are bit-exact against the oracle's (learned or random start); the certificate's open set is exactly
+the frames whose stage-1 distance exceeds N0 (Nr + c sqrt(Nr)); stage 2 anneals are the oracle's anneals R1 ..
+R1 + R2 - 1 of the open frames, bit for bit (so continuing with new Philox streams); the final choice is stage 1's
+unless a stage-2 candidate is strictly closer, within float32 near-ties of the oracle's distances."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print('[case]', *a, flush=True)
+
+
+@pytest.fixture(scope='module')
+def model():
+    from arpeggio_amd.learned import DEFAULT_WEIGHTS, WarmStartMLP
+    return WarmStartMLP.load(DEFAULT_WEIGHTS)
+
+
+@pytest.mark.parametrize('start,R1,L1,R2,L2,c,snr', [('learned', 4, 16, 8, 32, 3.0, 22.0),
+                                                      ('rand', 2, 8, 6, 24, 1.0, 20.0),
+                                                      ('learned', 3, 10, 5, 14, 4.0, 18.0)])
+def test_adaptive_vs_oracle(orc, model, start, R1, L1, R2, L2, c, snr):
+    from arpeggio_amd import frames
+    from arpeggio_amd.detector import AdaptivePipeline
+    say('adaptive', start, R1, L1, R2, L2, c, snr)
+    B, nt, M, key = 192, 32, 16, 0x5151AB
+    b = frames.draw(808, snr, 0, B, nt, M, dev='cuda')
+    lt = b['l'].to(torch.int8).contiguous()
+    pipe = AdaptivePipeline(B, 2 * nt, 2 * nt, M, R1=R1, L1=L1, R2=R2, L2=L2, c=c, start=start, key=key, inst0=0,
+                            model=model if start == 'learned' else None)
+    lhat, be = pipe.run(b['H'], b['y'], b['N0'], lt)
+    torch.cuda.synchronize()
+    G, h = pipe.Gi.cpu().numpy(), pipe.hi.cpu().numpy()
+    T1, T2 = pipe.T1t.cpu().numpy(), pipe.T2t.cpu().numpy()
+    inst = np.arange(B)
+    if start == 'learned':
+        thr = pipe.thr.cpu().numpy().view(np.uint32)
+        s1, de1, st1, used1 = orc.arpeggio_ws(G, h, 2, R1, T1, pipe.q, inst, key, thr, gate=True)
+        assert (pipe.used1.cpu().numpy() == used1).all()
+    else:
+        s1, de1, st1 = orc.arpeggio_fx(G, h, 2, R1, T1, pipe.q, inst, key, 'rand')
+    assert (pipe.spins1.cpu().numpy() == s1).all() and (pipe.de1.cpu().numpy() == de1).all()
+    assert (pipe.st1.cpu().numpy() == st1).all()
+    # stage-1 decision and distance
+    H, y, xg = b['H'].cpu().numpy(), b['y'].cpu().numpy(), pipe.xg.cpu().numpy()
+    Lc1 = orc.spins_to_levels_delta(s1, xg, 1, M)
+    allc1 = np.concatenate([Lc1, xg[:, None, :].astype(np.float64)], 1)
+    dd1 = orc.dist(H, y, allc1)
+    d1 = pipe.d1.cpu().numpy()
+    np.testing.assert_allclose(d1, dd1.min(1), rtol=1e-4, atol=1e-3)
+    # certificate: the open set is exactly the frames with d1 > N0 (Nr + c sqrt(Nr)) (on the GPU's own distances)
+    open_ = pipe.open_frames().numpy()
+    want = np.flatnonzero(~(d1 <= np.float32(b['N0']) * np.float32(pipe.thr_scale)))
+    assert np.array_equal(np.sort(open_), want)
+    say('open frames', len(want), 'of', B)
+    n = int(pipe.count.item())
+    slots = pipe.list[:n].cpu().numpy()
+    if n:
+        # stage 2 = the oracle's anneals R1 .. R1 + R2 - 1 of the open frames
+        if start == 'learned':
+            s12, de12, st12, used12 = orc.arpeggio_ws(G[slots], h[slots], 2, R1 + R2, T2, pipe.q, slots, key,
+                                                      thr[slots], gate=True)
+            assert (pipe.used2[:n].cpu().numpy() == used12[:, R1:]).all()
+        else:
+            s12, de12, st12 = orc.arpeggio_fx(G[slots], h[slots], 2, R1 + R2, T2, pipe.q, slots, key, 'rand')
+        assert (pipe.spins2[:n].cpu().numpy() == s12[:, R1:]).all()
+        assert (pipe.de2[:n].cpu().numpy() == de12[:, R1:]).all()
+        assert (pipe.st2[:n].cpu().numpy() == st12[:, R1:]).all()
+    # final choice per frame
+    lg = lhat.cpu().numpy()
+    d2 = pipe.d2.cpu().numpy()
+    opened = set(slots.tolist())
+    for i in range(B):
+        if i not in opened:
+            j = int(np.argmin(dd1[i]))
+            assert (lg[i] == allc1[i, j]).all() or dd1[i][(allc1[i] == lg[i]).all(1)].min() <= dd1[i].min() * (
+                1 + 1e-5) + 1e-6, i
+            continue
+        k = int(np.flatnonzero(slots == i)[0])
+        Lc2 = orc.spins_to_levels_delta(s12[k:k + 1, R1:], xg[i:i + 1], 1, M)[0]
+        allc = np.concatenate([allc1[i], Lc2], 0)
+        dd = orc.dist(H[i:i + 1], y[i:i + 1], allc[None])[0]
+        same = (allc == lg[i]).all(1)
+        assert same.any(), i
+        assert dd[same].min() <= dd.min() * (1 + 1e-5) + 1e-6, i
+        assert d2[i] <= d1[i]
+    assert (be.cpu().numpy() == orc.biterr_per_use(b['l'].cpu().numpy(), lg, M)).all()
+
+
+def test_adaptive_graph_replay(model):
+    """The whole two-stage step captured in a HIP graph replays to the eager result (list, count and all outputs),
+    with eager work between replays."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.detector import AdaptivePipeline
+    say('adaptive graph')
+    B = 512
+    b = frames.draw(809, 22.0, 0, B, 32, 16, dev='cuda')
+    lt = b['l'].to(torch.int8).contiguous()
+    pipe = AdaptivePipeline(B, 64, 64, 16, R1=4, L1=16, R2=8, L2=32, c=3.0, start='learned', key=3, model=model)
+    pipe.run(b['H'], b['y'], b['N0'], lt)
+    torch.cuda.synchronize()
+    op = pipe.open_frames()
+    ref = (pipe.lhat.clone(), pipe.biterr.clone(), op, pipe.d2[op.cuda()].clone())   # d2: written for open frames
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        pipe.run(b['H'], b['y'], b['N0'], lt)
+    for k in range(3):
+        pipe.lhat.zero_()
+        pipe.biterr.fill_(-1)
+        pipe.count.fill_(-5)
+        g.replay()
+        torch.cuda.synchronize()
+        got = (pipe.lhat, pipe.biterr, pipe.open_frames(), pipe.d2[op.cuda()])
+        for name, a, r in zip(('lhat', 'biterr', 'open', 'd2'), got, ref):
+            assert torch.equal(a.cpu(), r.cpu()), (k, name)
+        pipe.run(b['H'], b['y'], b['N0'], lt)
+        torch.cuda.synchronize()

@@ -140,8 +140,13 @@ def test_pipeline_chain_bit_exact(orc, policy, nt, rho, R):
         assert (pipe.spins.cpu().numpy() == s).all() and (pipe.e.cpu().numpy() == e).all()
     H, y = b['H'].cpu().numpy(), b['y'].cpu().numpy()
     Lc = orc.spins_to_levels_delta(s, xg, 1, M)
-    lh, _, d = orc.select(H, y, Lc, xg)
-    agree = (lhat.cpu().numpy() == lh).all(1)
-    assert agree.mean() >= 0.97
+    lh, allc, d = orc.select(H, y, Lc, xg)
+    lg = lhat.cpu().numpy()
+    # the GPU's choice is the oracle's, or a distinct candidate within float32 rounding of the oracle's minimum
+    for i in range(B):
+        if not (lg[i] == lh[i]).all():
+            same = (allc[i] == lg[i]).all(1)
+            assert same.any(), i
+            assert d[i][same].min() <= d[i].min() * (1 + 1e-5) + 1e-6, i
     be_o = orc.biterr_per_use(b['l'].cpu().numpy(), lhat.cpu().numpy(), M)
     assert (be.cpu().numpy() == be_o).all()
