@@ -144,10 +144,13 @@ class DetectPipeline:
                 raise ValueError('DetectPipeline.run: per-frame N0 must be a contiguous float32 GPU tensor of %d '
                                  'values' % B)
 
-    def run(self, H, y, N0, ltrue=None, stream=None, events=None):
+    STAGES = ('front_end', 'mlp', 'anneal', 'decode')
+
+    def run(self, H, y, N0, ltrue=None, stream=None, events=None, marks=None):
         """Launches front end, anneal and decode on one stream (default: torch's current stream; the MLP of the learned
         start runs on the same stream); returns (lhat, biterr or None) device tensors.
-        events: optional (start, end) torch.cuda.Event pair recorded on the same stream around the anneal launch."""
+        events: optional (start, end) torch.cuda.Event pair recorded on the same stream around the anneal launch.
+        marks: optional list of len(STAGES) + 1 events recorded before the front end and after each stage."""
         self._check_inputs(H, y, N0, ltrue)
         L, s = self.lib, _lib.stream_ptr(stream)
         ts = stream if stream is not None else torch.cuda.current_stream()
@@ -155,6 +158,11 @@ class DetectPipeline:
         def mark(i):
             if events is not None:
                 events[i].record(ts)
+
+        def stage(i):
+            if marks is not None:
+                marks[i].record(ts)
+        stage(0)
         B, Mr, D, M, R, P = self.B, self.Mr, self.D, self.M, self.R, self.P
         n0t, n0s = _n0_arg(N0, B)
         P_ = _lib.ptr
@@ -162,7 +170,9 @@ class DetectPipeline:
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
                                        P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None,
                                        None, None, P_(self.feat), self.feat.shape[1], s), 'front_end')
+            stage(1)
             self.model.thresholds(self.feat, self.inv_temp, out=self.thr, stream=stream)
+            stage(2)
             mark(0)
             _lib.check(L.arp_anneal_arpeggio_ws(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D,
                                                 B, P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
@@ -173,6 +183,8 @@ class DetectPipeline:
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
                                        P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None,
                                        None, None, None, 0, s), 'front_end')
+            stage(1)
+            stage(2)
             mark(0)
             _lib.check(L.arp_anneal_arpeggio(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D, B,
                                              P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
@@ -183,14 +195,18 @@ class DetectPipeline:
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg), None,
                                        None, P_(self.scale), None, None, P_(self.J), P_(self.hN), None, None, None,
                                        None, None, 0, s), 'front_end')
+            stage(1)
+            stage(2)
             mark(0)
             _lib.check(L.arp_anneal_chord(P_(self.J), P_(self.hN), P_(self.hN), P_(self.T), P_(self.c), self.q, self.S,
                                           P, R, P * D, B, self.key & MASK, (self.key >> 32) & MASK, self.inst0, 0,
                                           P_(self.e), P_(self.spins), None, s), 'chord')
             mark(1)
+        stage(3)
         _lib.check(L.arp_decode_select(P_(H), P_(y), P_(self.xg), P_(self.spins), B, R, P, D, Mr, M, 1, P_(ltrue),
                                        P_(self.lhat), P_(self.biterr) if ltrue is not None else None, None, s),
                    'decode_select')
+        stage(4)
         return self.lhat, (self.biterr if ltrue is not None else None)
 
 
@@ -269,12 +285,20 @@ class AdaptivePipeline:
 
     _check_inputs = DetectPipeline._check_inputs
 
-    def run(self, H, y, N0, ltrue=None, stream=None, events=None):
+    STAGES = ('front_end', 'mlp', 'anneal_1', 'decode_1', 'certify', 'anneal_2', 'decode_2')
+
+    def run(self, H, y, N0, ltrue=None, stream=None, events=None, marks=None):
         """Both stages on one stream; returns (lhat, biterr or None). events: optional (start, end) pair recorded around
-        the two anneal stages (certificate included)."""
+        the two anneal stages (certificate included). marks: optional list of len(STAGES) + 1 events recorded before the
+        front end and after each stage (for per-stage times)."""
         self._check_inputs(H, y, N0, ltrue)
         L, s = self.lib, _lib.stream_ptr(stream)
         ts = stream if stream is not None else torch.cuda.current_stream()
+
+        def mark(i):
+            if marks is not None:
+                marks[i].record(ts)
+        mark(0)
         B, Mr, D, M, P = self.B, self.Mr, self.D, self.M, self.P
         n0t, n0s = _n0_arg(N0, B)
         P_ = _lib.ptr
@@ -283,8 +307,10 @@ class AdaptivePipeline:
                                    P_(self.hi), P_(self.scale), None, None, None, None, None, None, None, None,
                                    P_(self.feat) if learned else None, self.feat.shape[1] if learned else 0, s),
                    'front_end')
+        mark(1)
         if learned:
             self.model.thresholds(self.feat, self.inv_temp, out=self.thr, stream=stream)
+        mark(2)
         if events is not None:
             events[0].record(ts)
         k0, k1 = self.key & MASK, (self.key >> 32) & MASK
@@ -293,23 +319,28 @@ class AdaptivePipeline:
                                               self.R1, D, B, P_(self.inst), k0, k1, 0, P_(self.thr), gate, 0, None,
                                               None, P_(self.spins1), P_(self.de1), P_(self.st1),
                                               P_(self.used1) if learned else None, s), 'anneal stage 1')
+        mark(3)
         lt = P_(ltrue)
         _lib.check(L.arp_decode_select_list(P_(H), P_(y), P_(self.xg), P_(self.spins1), B, self.R1, P, D, Mr, M, 1,
                                             lt, None, None, None, None, P_(self.lhat),
                                             P_(self.biterr) if ltrue is not None else None, P_(self.d1), s),
                    'decode stage 1')
+        mark(4)
         _lib.check(L.arp_certify(P_(self.d1), P_(n0t), n0s, self.thr_scale, B, P_(self.list), P_(self.count), s),
                    'certify')
+        mark(5)
         _lib.check(L.arp_anneal_arpeggio_list(P_(self.Gi), P_(self.hi), P_(self.T2t), self.T2t.numel(), self.q, P,
                                               self.R2, D, B, P_(self.inst), k0, k1, 0, P_(self.thr), gate, self.R1,
                                               P_(self.list), P_(self.count), P_(self.spins2), P_(self.de2),
                                               P_(self.st2), P_(self.used2) if learned else None, s), 'anneal stage 2')
+        mark(6)
         if events is not None:
             events[1].record(ts)
         _lib.check(L.arp_decode_select_list(P_(H), P_(y), P_(self.xg), P_(self.spins2), B, self.R2, P, D, Mr, M, 1,
                                             lt, P_(self.list), P_(self.count), P_(self.lhat), P_(self.d1),
                                             P_(self.lhat), P_(self.biterr) if ltrue is not None else None,
                                             P_(self.d2), s), 'decode stage 2')
+        mark(7)
         return self.lhat, (self.biterr if ltrue is not None else None)
 
     def open_frames(self):

@@ -1,24 +1,30 @@
 #!/usr/bin/env python3
 """Detections/s at fixed BER on one MI355X, learned-start vs plain annealer: 32 users x 32 rx, 16-QAM, i.i.d.
-Rayleigh, 16 anneals per frame, 65,536 frames per step, one-stage dDI (K = 1).
+Rayleigh, 65,536 frames per step, one-stage dDI (K = 1).
 
-A step is one pass of the kernel path over the batch with H, y and the true levels already in HBM:
-front end (LMMSE + dDI Ising build + fixed point [+ warm-start features]) -> [MLP -> start thresholds] ->
-fixed-point Arpeggio anneal (first accept) -> decode/select -> bit errors.
+A step is one pass of the kernel path over the batch with H, y and the true levels already in HBM.
+  plain:    front end (LMMSE + dDI Ising build + fixed point) -> fixed-point Arpeggio (first accept), 16 anneals x 64
+            passes from the random start -> decode/select -> bit errors (the reference's 32-user pick
+            abl_far_32_0.1_0.1_0_rand_*: 32 sweeps x 2 copies, T = 0.1, q = 0)
+  learned:  the adaptive detector: front end with the warm-start features -> MLP -> start thresholds -> stage 1, 4
+            learned-start anneals x 24 passes for every frame -> decode -> certificate ||y - Hx||^2 <= N0 (Nr + c
+            sqrt(Nr)) (the reference's first-pass stop rule, arpeggio/sim/a56an.py:58-67) -> stage 2, the plain
+            budget (16 anneals x 64 passes, new streams) for the open frames only -> decode against stage 1's choice
 
 Procedure (fixed-BER comparison):
-  1. plain = random start, 64 passes (32 sweeps x 2 copies), T = 0.1, q = 0 (the reference's 32-user pick
-     abl_far_32_0.1_0.1_0_rand_*); the SNR where plain reaches BER 1e-3 is found on calibration frames
-     (log-BER interpolated between bracketing SNRs; --snr skips this);
-  2. on the measurement frames at that SNR, the learned start runs at each budget of the ladder; the chosen
-     budget is the smallest whose paired-bootstrap BER ratio to plain (95 % CI) reaches down to 1.0;
-  3. both pipelines are timed over K steps; value = learned-start detections/s at the chosen budget,
-     plain.value and speedup beside it.
+  1. the SNR where plain reaches BER 1e-3 is found on calibration frames (log-BER interpolated between bracketing
+     SNRs; --snr skips this);
+  2. c is picked by the reference's rule (a56an.py:109-118) on the calibration frames at that SNR: the largest c of
+     its grid whose paired-bootstrap BER ratio to the same detector without the stop has an upper 95 % bound <= 1.10;
+  3. on the measurement frames, the learned detector's BER must match plain's: paired-bootstrap ratio (a56an.py:74-95)
+     with an upper 95 % bound <= 1.10 (the reference's equivalence cut-off, README.md:171);
+  4. both pipelines are timed over K steps; value = learned detections/s, plain.value and speedup beside it, and the
+     same adaptive detector from the random start (what the network adds) and the learned start without the stop.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--snr DB] [--frames B] [--no-cpu] [--with-chord]
 
-Extra fields: roofline (the timed step's anneal kernel, HIP events on its own stream), cpu_baseline (the C oracle
-of the plain chain on a bounded sample, on this host's cores)."""
+Extra fields: roofline (the timed step's dominant kernel, HIP events on its own stream), stages (per-stage ms),
+cpu_baseline (the C oracle of the plain chain on this host's cores, three timed segments)."""
 import argparse
 import json
 import math
@@ -36,6 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # delivers 128 B per clock per CU
 SIMDS, CLOCK_HZ = 256 * 4, 2.4e9
 LDS_PEAK_GBS = 128 * 256 * CLOCK_HZ / 1e9
+MFMA_FP32_TFLOPS = 157.3
 NT, M = 32, 16
 METRIC = 'detections/s at fixed BER (32u 16-QAM), learned-start vs plain annealer'
 
@@ -51,13 +58,13 @@ def parse():
     ap.add_argument('--target-ber', type=float, default=1e-3)
     ap.add_argument('--copies', type=int, default=16)
     ap.add_argument('--passes', type=int, default=64, help='plain annealer pass budget')
-    ap.add_argument('--ladder', default='16,32,48,56,64,96,128,256',
-                    help='learned-start pass budgets tried, ascending (multiples of 8)')
+    ap.add_argument('--stage1', default='4,24', help='adaptive stage 1: anneals,passes')
+    ap.add_argument('--c', type=float, default=None, help='certificate constant (default: picked on calibration '
+                    'frames by the reference rule)')
     ap.add_argument('--weights', default=None)
     ap.add_argument('--seed', type=int, default=1)
     ap.add_argument('--plain-only', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--cpu-frames', type=int, default=65536)
     ap.add_argument('--warm-seconds', type=float, default=1.0, help='minimum warm-up time before each timed loop')
     ap.add_argument('--with-chord', action='store_true')
     ap.add_argument('--no-batched', action='store_true', help='skip the mixed-size batch measurement')
@@ -122,6 +129,57 @@ def calibrate(args):
     return round(s0 + t * (s1 - s0), 2), pts
 
 
+C_GRID = (0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0)   # the reference's grid of stop constants (a56an.py:10)
+EQUIV_LIMIT = 1.10                             # the reference's BER-equivalence cut-off (README.md:171)
+
+
+def make_adaptive(B, args, start, model=None, c=3.0):
+    from arpeggio_amd.detector import AdaptivePipeline
+    R1, L1 = (int(x) for x in args.stage1.split(','))
+    return AdaptivePipeline(B, 2 * NT, 2 * NT, M, R1=R1, L1=L1, R2=args.copies, L2=args.passes, c=c, start=start,
+                            T0=0.1, T1=0.1, q=0.0, key=key_of(args.seed), inst0=0,
+                            model=model if start == 'learned' else None)
+
+
+def pick_c(args, snr, model):
+    """The reference's pick_c (arpeggio/sim/a56an.py:109-118) on calibration frames (draw index 1) at the measurement
+    SNR: the largest c of its grid whose paired BER ratio to the same detector without the stop (every frame takes
+    stage 2) has an upper 95 % bound <= 1.10."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.stats import boot_ratio
+    Bc = args.calib_frames
+    b = frames.draw(args.seed, snr, 1, Bc, NT, M, dev='cuda')
+    pipe = make_adaptive(Bc, args, 'learned', model, c=0.0)
+    pipe.thr_scale = -1.0                       # no stop: every frame is open
+    base = frame_errors(pipe, b)
+    rows, best = [], None
+    for c in C_GRID:
+        pipe.c = c
+        pipe.thr_scale = NT + c * NT ** 0.5
+        be = frame_errors(pipe, b)
+        r, lo, hi = boot_ratio(be, base)
+        rows.append(dict(c=c, ratio=r, ci=[lo, hi], open=int(pipe.count.item()) / Bc))
+        log('stop constant c = %.1f: BER ratio to no stop %.3f [%.3f, %.3f], open %.3f' % (c, r, lo, hi,
+                                                                                          rows[-1]['open']))
+        if hi is not None and hi <= EQUIV_LIMIT:
+            best = c
+    return best, rows
+
+
+def stage_ms(pipe, b, reps=5):
+    """Mean per-stage times (HIP events on the launch stream between the stages) over reps steps."""
+    import torch
+    lt = b['l'].to(torch.int8).contiguous()
+    n = len(pipe.STAGES) + 1
+    acc = np.zeros(n - 1)
+    for _ in range(reps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+        pipe.run(b['H'], b['y'], b['N0'], lt, marks=ev)
+        torch.cuda.synchronize()
+        acc += [ev[i].elapsed_time(ev[i + 1]) for i in range(n - 1)]
+    return {k: float(v) for k, v in zip(pipe.STAGES, acc / reps)}
+
+
 def time_pipe(pipe, b, steps, warmup, min_warm_s=1.0):
     """K timed steps after at least W warm-up steps and at least min_warm_s seconds of warm-up work (the GPU's clocks
     settle under load); barrier = synchronize on both sides. Returns det/s, ms/step, mean anneal kernel ms (HIP
@@ -156,84 +214,106 @@ def time_pipe(pipe, b, steps, warmup, min_warm_s=1.0):
                 ber=float(errs.item()) / (B * 2 * pipe.D * steps))
 
 
-def cpu_baseline(b, pipe, B_cpu, args):
-    """The oracle's plain chain on the first B_cpu frames on the host cores: numpy float64 front end, the C
-    fixed-point Arpeggio (threads = min(16, cores)) on the GPU front end's integers, numpy decode. Wall time ->
-    frames/s."""
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_baseline(b, pipe, args, seg_s=6.0, nseg=3):
+    """The oracle's plain chain on the host: numpy float64 front end, the C fixed-point Arpeggio (one thread per core of
+    the GPU's CPU share: 16 on the one-GPU box) on the GPU front end's integers, numpy decode/select. Timed as nseg
+    segments of at least seg_s seconds each, each segment detecting chunks of 2,048 of the step's frames in turn (the
+    reference's segment rule, arpeggio/mimo/bench_b.py:63-90); value = detections / wall time over all segments."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle
     cores = min(16, os.cpu_count() or 1)
-    H = b['H'][:B_cpu].cpu().numpy()
-    y = b['y'][:B_cpu].cpu().numpy()
-    G = pipe.Gi[:B_cpu].cpu().numpy()
-    h = pipe.hi[:B_cpu].cpu().numpy()
+    chunk = 2048
+    H = b['H'].cpu().numpy()
+    y = b['y'].cpu().numpy()
+    G = pipe.Gi.cpu().numpy()
+    h = pipe.hi.cpu().numpy()
     T = pipe.T.cpu().numpy()
-    inst = np.arange(B_cpu)
+    gs = pipe.spins.cpu().numpy()
     oracle.lib()
-    t0 = time.perf_counter()
-    z, Ainv, _ = oracle.mmse(H, y, b['N0'], M)
-    xg = oracle.mmse_hard(H, y, b['N0'], M, z, Ainv)
-    oracle.ising_delta_at(H, y, xg, 1)
-    s, de, st = oracle.arpeggio_fx(G, h, 2, args.copies, T, pipe.q, inst, pipe.key, 'rand', nthreads=cores)
-    Lc = oracle.spins_to_levels_delta(s, xg, 1, M)
-    oracle.select(H, y, Lc, xg)
-    dt = time.perf_counter() - t0
-    gs = pipe.spins[:B_cpu].cpu().numpy()
-    agree = bool((gs == s).all())
-    return dict(value=B_cpu / dt, unit='detections/s', cores=cores, kind='port',
-                sample='first %d frames of the plain step: float64 numpy front end, C oracle Arpeggio (%d threads, '
-                       '%d passes, R=%d) on the same integer inputs, numpy decode/select; %.1f s wall; anneal spins '
-                       'equal the GPU: %s' % (B_cpu, cores, args.passes, args.copies, dt, agree))
+    B = H.shape[0]
+    segs, agree, done, pos = [], True, 0, 0
+    for _ in range(nseg):
+        t0, n = time.perf_counter(), 0
+        while time.perf_counter() - t0 < seg_s:
+            sl = slice(pos, pos + chunk)
+            z, Ainv, _ = oracle.mmse(H[sl], y[sl], b['N0'], M)
+            xg = oracle.mmse_hard(H[sl], y[sl], b['N0'], M, z, Ainv)
+            oracle.ising_delta_at(H[sl], y[sl], xg, 1)
+            s, de, st = oracle.arpeggio_fx(G[sl], h[sl], 2, args.copies, T, pipe.q, np.arange(pos, pos + chunk),
+                                           pipe.key, 'rand', nthreads=cores)
+            Lc = oracle.spins_to_levels_delta(s, xg, 1, M)
+            oracle.select(H[sl], y[sl], Lc, xg)
+            agree &= bool((gs[sl] == s).all())
+            n += chunk
+            pos = (pos + chunk) % B
+        segs.append(dict(frames=n, seconds=time.perf_counter() - t0))
+        done += n
+    wall = sum(x['seconds'] for x in segs)
+    rates = [x['frames'] / x['seconds'] for x in segs]
+    return dict(value=done / wall, unit='detections/s', cores=cores, kind='port',
+                nproc=os.cpu_count(), cpu_model=cpu_model(), segments=segs, segment_rates=rates,
+                sample='the plain step on the host: chunks of %d of its frames (float64 numpy front end, C oracle '
+                       'Arpeggio with %d threads, %d passes, R = %d, on the GPU front end\'s integers, numpy decode) in '
+                       '%d segments of >= %.0f s; %d detections in %.1f s; anneal spins equal to the GPU\'s: %s' %
+                       (chunk, cores, args.passes, args.copies, nseg, seg_s, done, wall, agree))
 
 
-def roofline(pipe, t, B, L, R, learned):
-    """Dominant kernel of the timed step = the anneal. Algorithmic HBM bytes per launch: G (int32, D x D) + h + inst
-    per frame and T per pass in [+ start thresholds P D int32 per frame]; spins (R P D int8), best energy (int64),
-    (flips, tests) (2 int64) [+ used (int8)] per anneal out."""
+def roofline_front_end(t_ms, B):
+    """The front end: LMMSE + Gram on the matrix cores. Algorithmic work per frame (the Gram, H^T H, and H^T y):
+    2 (2Nr)(2Nt)^2 + 2 (2Nr)(2Nt) FLOP; the bound is the FP32 MFMA peak."""
+    D = Mr = 2 * NT
+    flop = B * (2 * Mr * D * D + 2 * Mr * D)
+    achieved = flop / (t_ms * 1e-3) / 1e12
+    hbm = B * (4 * Mr * D + 4 * Mr + 4 * D * D + 4 * D + D + 4 + 2 * feat_len())
+    return dict(bound='mfma', achieved=achieved, peak=MFMA_FP32_TFLOPS, unit='TFLOP/s',
+                frac=achieved / MFMA_FP32_TFLOPS, traffic=None, kernel='front_end_w64_kernel<64>', launch_ms=t_ms,
+                algorithmic_flop=flop, hbm_algorithmic_bytes=hbm,
+                hbm=dict(achieved_GBs=hbm / (t_ms * 1e-3) / 1e9, peak_GBs=HBM_PEAK_GBS,
+                         frac=hbm / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
+
+
+def roofline_anneal(t_ms, B, R, L, flips, learned, kernel):
+    """A first-accept anneal launch over B frames, R anneals of L passes each. Algorithmic HBM bytes per launch: G
+    (int32, D x D) + h + inst per frame and T per pass in [+ start thresholds P D int32 per frame]; spins (R P D int8),
+    best energy (int64), (flips, tests) (2 int64) [+ kept flag] per anneal out. The bound that holds is instruction
+    issue; the LDS bytes (one Gram column per flip, the D x D initial-field sweep) are reported beside it."""
     D, P = 2 * NT, 2
     hbm = B * (4 * D * D + 4 * D + 4) + 4 * L + B * R * (P * D + 8 + 16)
     if learned:
         hbm += B * 4 * P * D + B * R
-    ms = t['anneal_ms']
-    achieved = hbm / (ms * 1e-3) / 1e9
-    st = pipe.st.double()
-    flips = float(st[..., 0].mean())
+    achieved = hbm / (t_ms * 1e-3) / 1e9
     lds = B * R * 4 * D * (flips + D)
-    traffic = None
-    kern = 'arpeggio_fx_kernel<2,1,0,true,64>'
-    tp = os.path.join(ROOT, 'profiles', 'arpeggio_anneal_traffic.json')
-    if os.path.exists(tp):
-        for rec in json.load(open(tp)).get('launches', []):
-            if (rec.get('frames'), rec.get('passes'), rec.get('copies'), rec.get('learned')) == (B, L, R, learned):
-                traffic = rec.get('hbm_bytes_per_launch')
-    # the bound that holds: vector-instruction issue. Vector quad-cycles per anneal-pass come from the SQ counters
-    # of the same launch shape (SQ_ACTIVE_INST_VALU / anneal-passes); the SIMD cycles are this launch's own time.
     issue = None
     ip = os.path.join(ROOT, 'profiles', 'arpeggio_anneal_issue.json')
     if os.path.exists(ip):
         for rec in json.load(open(ip)).get('launches', []):
             if rec.get('start') == ('learned' if learned else 'rand'):
-                q = rec['valu_quad_cycles_per_anneal_pass']
-                qs = rec['salu_quad_cycles_per_anneal_pass']
-                busy = 4 * q * B * R * L
-                avail = ms * 1e-3 * SIMDS * CLOCK_HZ
+                q, qs = rec['valu_quad_cycles_per_anneal_pass'], rec['salu_quad_cycles_per_anneal_pass']
+                avail = t_ms * 1e-3 * SIMDS * CLOCK_HZ
                 issue = dict(bound='instruction issue (vector and scalar)', valu_quad_cycles_per_anneal_pass=q,
-                             salu_quad_cycles_per_anneal_pass=qs,
-                             achieved_simd_cycles=busy, available_simd_cycles=avail, frac=busy / avail,
+                             salu_quad_cycles_per_anneal_pass=qs, frac=4 * q * B * R * L / avail,
                              salu_frac=4 * qs * B * R * L / avail,
                              counters_at=dict(frames=rec['frames'], copies=rec['copies'], passes=rec['passes']),
                              source='profiles/arpeggio_anneal_issue.json')
-    lds_gbs = lds / (ms * 1e-3) / 1e9
     return dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
-                traffic=traffic, kernel=kern + (' (learned start)' if learned else ''), algorithmic_bytes=hbm,
-                launch_ms=ms, flips_per_anneal=flips,
-                lds=dict(algorithmic_bytes=lds, achieved_GBs=lds_gbs, peak_GBs=LDS_PEAK_GBS,
-                         frac=lds_gbs / LDS_PEAK_GBS),
-                issue=issue,
-                note='the anneal is an on-chip loop bound by instruction issue (Philox4x32-10, the 64-bit decision, '
-                     'the ballot scan with its scalar find-first-set chain): issue.frac and issue.salu_frac are the '
-                     'fractions of SIMD cycles spent issuing its vector and scalar instructions; HBM and LDS bytes '
-                     'are the algorithmic ones')
+                traffic=None, kernel=kernel, algorithmic_bytes=hbm, launch_ms=t_ms, flips_per_anneal=flips,
+                lds=dict(algorithmic_bytes=lds, achieved_GBs=lds / (t_ms * 1e-3) / 1e9, peak_GBs=LDS_PEAK_GBS,
+                         frac=lds / (t_ms * 1e-3) / 1e9 / LDS_PEAK_GBS),
+                issue=issue)
+
+
+def feat_len():
+    return (4 * 2 * NT + 1 + 31) // 32 * 32
 
 
 def batched(args):
@@ -324,53 +404,83 @@ def main():
     nbits = B * 4 * NT
     out = dict(metric=METRIC, unit='detections/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
                higher_is_better=True, scaling='weak', vs_baseline=None, dtype='int32', data='synthetic')
-    ladder, chosen, model = [], None, None
+    model = None
+    log('timing plain (%d anneals x %d passes)' % (R, L))
+    tp = time_pipe(plain, b, args.steps, args.warmup, args.warm_seconds)
+    plain_rec = dict(value=tp['value'], ms_per_step=tp['ms_per_step'], gpu_ms_per_step=tp['gpu_ms_per_step'],
+                     passes=L, copies=R, start='rand', ber=float(be_plain.sum()) / nbits, anneal_ms=tp['anneal_ms'],
+                     stages=stage_ms(plain, b))
+    matched = False
     if not args.plain_only:
         from arpeggio_amd.learned import DEFAULT_WEIGHTS, WarmStartMLP
         model = WarmStartMLP.load(args.weights or DEFAULT_WEIGHTS)
-        for Ll in [int(x) for x in args.ladder.split(',')]:
-            pl = make_pipe(B, args, Ll, 'learned', model)
-            be = frame_errors(pl, b)
+        if args.c is None:
+            c, crows = pick_c(args, snr, model)
+            if c is None:
+                c = C_GRID[0]
+        else:
+            c, crows = args.c, None
+        log('stop constant c = %.1f' % c)
+        recs = {}
+        for start in ('learned', 'rand'):
+            ap = make_adaptive(B, args, start, model, c)
+            be = frame_errors(ap, b)
             r, lo, hi = boot_ratio(be, be_plain)
-            rec = dict(passes=Ll, ber=float(be.sum()) / nbits, ratio=r, ci=[lo, hi],
-                       kept_learned=float(pl.used.double().mean()))
-            log('learned start, %d passes: BER %.3e, ratio to plain %.3f [%.3f, %.3f]' % (Ll, rec['ber'], r, lo, hi))
-            ladder.append(rec)
-            del pl
-            if lo <= 1.0:
-                chosen = Ll
-                break
-    log('timing plain (%d passes)' % L)
-    tp = time_pipe(plain, b, args.steps, args.warmup, args.warm_seconds)
-    plain_rec = dict(value=tp['value'], ms_per_step=tp['ms_per_step'], gpu_ms_per_step=tp['gpu_ms_per_step'],
-                     passes=L, start='rand', ber=float(be_plain.sum()) / nbits, anneal_ms=tp['anneal_ms'])
-    if chosen is not None:
-        log('timing learned start (%d passes)' % chosen)
-        lp = make_pipe(B, args, chosen, 'learned', model)
-        tl = time_pipe(lp, b, args.steps, args.warmup, args.warm_seconds)
+            open_frac = int(ap.count.item()) / B
+            log('adaptive, %s start: BER %.3e, ratio to plain %.3f [%.3f, %.3f], open %.3f' % (
+                start, be.sum() / nbits, r, lo, hi, open_frac))
+            t = time_pipe(ap, b, args.steps, args.warmup, args.warm_seconds)
+            recs[start] = dict(value=t['value'], ms_per_step=t['ms_per_step'], gpu_ms_per_step=t['gpu_ms_per_step'],
+                               ber=t['ber'], ratio_to_plain=r, ci=[lo, hi], ber_matched=bool(hi <= EQUIV_LIMIT),
+                               open_fraction=open_frac, stage1=args.stage1, stage2='%d,%d' % (R, L), c=c,
+                               speedup_vs_plain=t['value'] / tp['value'], stages=stage_ms(ap, b))
+            if start == 'learned':
+                lp = ap
+                tl = t
+            else:
+                del ap
+        # the learned start at the plain budget without the stop (what the start alone does)
+        ns = make_pipe(B, args, L, 'learned', model)
+        be = frame_errors(ns, b)
+        r, lo, hi = boot_ratio(be, be_plain)
+        recs['learned_no_stop'] = dict(ber=float(be.sum()) / nbits, ratio_to_plain=r, ci=[lo, hi], passes=L,
+                                       copies=R, kept_learned=float(ns.used.double().mean()))
+        del ns
+        learned = recs['learned']
+        matched = learned['ber_matched']
+        out['learned'] = learned
+        out['adaptive_random_start'] = recs['rand']
+        out['learned_no_stop'] = recs['learned_no_stop']
+        out['stop_constant'] = dict(c=c, grid=crows, rule='largest c with paired BER ratio to no stop <= 1.10 at the '
+                                    'upper 95 % bound (arpeggio/sim/a56an.py:109-118)')
+    if matched:
         out.update(value=tl['value'], ms_per_step=tl['ms_per_step'])
-        out['learned'] = dict(passes=chosen, ms_per_step=tl['ms_per_step'], gpu_ms_per_step=tl['gpu_ms_per_step'],
-                              ber=tl['ber'], anneal_ms=tl['anneal_ms'], ber_matched=True)
         out['speedup_vs_plain'] = tl['value'] / tp['value']
-        out['roofline'] = roofline(lp, tl, B, chosen, R, True)
-        timed = 'learned'
+        st = out['learned']['stages']
+        timed = 'learned (adaptive)'
+        if st['front_end'] >= st['anneal_1']:
+            out['roofline'] = roofline_front_end(st['front_end'], B)
+        else:
+            fl = float(lp.st1.double()[..., 0].mean())
+            out['roofline'] = roofline_anneal(st['anneal_1'], B, lp.R1, lp.L1, fl, True,
+                                              'arpeggio_fx_kernel<2,1,0,true,64> (stage 1, learned start)')
     else:
-        # no budget matched (or --plain-only): the plain annealer is the measured step
+        # no matched learned detector (or --plain-only): the plain annealer is the measured step
         out.update(value=tp['value'], ms_per_step=tp['ms_per_step'])
-        if not args.plain_only:
-            out['learned'] = dict(passes=None, ber_matched=False)
-        out['roofline'] = roofline(plain, tp, B, L, R, False)
         timed = 'plain'
+        fl = float(plain.st.double()[..., 0].mean())
+        out['roofline'] = roofline_anneal(plain_rec['stages']['anneal'], B, R, L, fl, False,
+                                          'arpeggio_fx_kernel<2,1,0,true,64>')
     out['plain'] = plain_rec
-    out['ladder'] = ladder
-    out['config'] = dict(workload='one-stage dDI detection, 32x32 16-QAM i.i.d. Rayleigh, %d frames/step, R=%d '
-                                  'anneals/frame; %s step timed' % (B, R, timed),
-                         frames=B, users=NT, rx=NT, M=M, copies=R, plain_passes=L, learned_passes=chosen,
+    out['config'] = dict(workload='one-stage dDI detection, 32x32 16-QAM i.i.d. Rayleigh, %d frames/step; plain: R=%d '
+                                  'anneals x %d passes; learned: adaptive, stage 1 %s (anneals,passes), stage 2 the '
+                                  'plain budget on open frames; %s step timed' % (B, R, L, args.stage1, timed),
+                         frames=B, users=NT, rx=NT, M=M, copies=R, plain_passes=L, stage1=args.stage1,
                          policy='arpeggio', snr_db=snr, target_ber=args.target_ber, calibration=calib,
                          T0=0.1, T1=0.1, q=0.0)
     if not args.no_cpu:
-        log('cpu baseline on %d frames' % args.cpu_frames)
-        out['cpu_baseline'] = cpu_baseline(b, plain, min(args.cpu_frames, B), args)
+        log('cpu baseline (3 segments)')
+        out['cpu_baseline'] = cpu_baseline(b, plain, args)
     if not args.no_64u:
         log('64 users, correlated channel (%d frames)' % args.frames_64u)
         out['users64'] = users64(args)
