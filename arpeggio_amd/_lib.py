@@ -13,6 +13,13 @@ _i32, _u32, _u64 = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64
 _l, _f = ctypes.c_long, ctypes.c_float
 
 # name -> argtypes; every entry point returns int (0 = success)
+class MlpWeights(ctypes.Structure):
+    """arp_mlp_weights: the warm-start MLP's device weights for the whole-chain entry point."""
+    _fields_ = [('W1t', ctypes.c_void_p), ('b1', ctypes.c_void_p), ('W2t', ctypes.c_void_p), ('b2', ctypes.c_void_p),
+                ('W3t', ctypes.c_void_p), ('b3', ctypes.c_void_p), ('F', ctypes.c_int), ('H', ctypes.c_int),
+                ('inv_temp', ctypes.c_float)]
+
+
 class BatchClass(ctypes.Structure):
     """arp_batch_class: one size class of a mixed batch (device pointers and sizes)."""
     _fields_ = [('Gt', ctypes.c_void_p), ('h', ctypes.c_void_p), ('inst', ctypes.c_void_p), ('T', ctypes.c_void_p),
@@ -51,7 +58,12 @@ SIGNATURES = {
     'arp_decode_select_list': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _P, _P, _P, _P, _P, _P, _P,
                                _P, _P],
     'arp_certify': [_P, _P, _f, _f, _i32, _P, _P, _P],
+    'arp_detect_adaptive': [_P, _P, _P, _f, _i32, _i32, _i32, _i32, _P, _i32, _i32, _P, _i32, _i32, _i32, _f, _P, _u32,
+                            _u32, _P, _i32, _P, _P, _P, _P, ctypes.c_size_t, _P],
 }
+
+# entry points whose return type is not int (bound in load())
+SPECIAL = {'arp_detect_workspace_bytes'}
 
 _lib = None
 
@@ -72,6 +84,8 @@ def load(path=None):
     L.arp_last_error.restype = ctypes.c_char_p
     L.arp_last_error.argtypes = []
     L.arp_abi_version.restype = ctypes.c_int
+    L.arp_detect_workspace_bytes.restype = ctypes.c_size_t
+    L.arp_detect_workspace_bytes.argtypes = [_i32, _i32, _i32, _i32, _i32]
     for name, args in SIGNATURES.items():
         f = getattr(L, name)
         f.argtypes = args

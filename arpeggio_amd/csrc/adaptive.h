@@ -1,6 +1,7 @@
 // Optional arguments of the first-accept and decode kernels for frame-list (compacted) launches, and the launch
 // wrappers of the early-stop path (certificate + compaction). Included by the kernels that take them and by the C ABI.
 #pragma once
+#include "../../include/arpeggio_hip.h"
 #include "common.h"
 
 namespace arp {
@@ -34,6 +35,16 @@ hipError_t launch_arpeggio_ex(const int32_t* Gt, const int32_t* h, const int32_t
 hipError_t launch_decode_ex(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R, int P,
                             int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat, int32_t* biterr,
                             float* dist_out, const DecodeExtra& ex, hipStream_t st);
+// The C ABI's error state (capi.hip): set_error(0, "") clears it and returns ARP_OK; hip_status maps a HIP error.
+int set_error(int code, const char* msg);
+int hip_status(hipError_t e, const char* what);
+
+// The whole chain of arp_detect_adaptive on a caller's workspace (detect.hip).
+int detect_chain(const float* H, const float* y, const float* N0, float N0_scalar, int B, int Mr, int D, int M,
+                 const int32_t* T1, int L1, int R1, const int32_t* T2, int L2, int R2, int q, float thr_scale,
+                 const uint32_t* inst, uint32_t k0, uint32_t k1, const arp_mlp_weights* mlp, int gate,
+                 const int8_t* ltrue, int8_t* lhat, int32_t* biterr, void* work, size_t work_bytes, void* stream);
+
 hipError_t launch_certify(const float* best_d, const float* N0v, float N0s, float thr_scale, int B, int32_t* list,
                           int32_t* count, hipStream_t st);
 

@@ -65,6 +65,11 @@ static int arpeggio_waves(int R, int D) {
 // A workgroup of `waves` wave64s must fit the kernel's compiled bound.
 static bool block_fits(int waves, int max_threads) { return waves >= 1 && ARP_WAVE * waves <= max_threads; }
 
+namespace arp {
+int set_error(int code, const char* msg) { return code == ARP_OK ? ok() : fail(code, "%s", msg); }
+int hip_status(hipError_t e, const char* what) { return hip_check(e, what); }
+}  // namespace arp
+
 extern "C" const char* arp_last_error(void) { return g_err.c_str(); }
 
 extern "C" int arp_abi_version(void) { return 2; }

@@ -9,6 +9,8 @@
 
 Reference call stack this replaces: arpeggio/mimo/fxdetect.py:24-43 (MMSE -> mapping -> to_fixed -> kernel ->
 levels -> select) with the fixed-point Arpeggio solver of arpeggio/sim/local_code_a513/run_a513.py:107-128."""
+import ctypes
+
 import torch
 
 from . import _lib
@@ -347,3 +349,42 @@ class AdaptivePipeline:
         """Host copy of the frame indices stage 2 annealed (sorted)."""
         n = int(self.count.item())
         return torch.sort(self.list[:n]).values.cpu()
+
+
+def detect_chain(H, y, N0, M, R1=4, L1=24, R2=16, L2=64, c=3.0, model=None, gate=True, key=0, inst0=0, T0=0.1,
+                 T1=0.1, q=0.0, ltrue=None, Q=QSCALE, work=None):
+    """The whole chain through the single C entry point arp_detect_adaptive (R2 = 0: one stage). Same results as
+    AdaptivePipeline (DetectPipeline for R2 = 0) on the same arguments. Returns (lhat, biterr or None)."""
+    H, y = _f32(H), _f32(y)
+    B, Mr, D = H.shape
+    L = _lib.load()
+    T1l, qi = arpeggio_schedule(L1 // 2, T0, T1, q, 1, Q)
+    schedule_check(T1l, qi, 2, D, Q)
+    T1t = torch.tensor(T1l, dtype=torch.int32, device='cuda')
+    T2t = None
+    if R2 > 0:
+        T2l, _ = arpeggio_schedule(L2 // 2, T0, T1, q, 1, Q)
+        schedule_check(T2l, qi, 2, D, Q)
+        T2t = torch.tensor(T2l, dtype=torch.int32, device='cuda')
+    inst = ((torch.arange(B, dtype=torch.int64) + inst0) & MASK).to(torch.int32).to('cuda')
+    n0t, n0s = _n0_arg(N0, B)
+    need = L.arp_detect_workspace_bytes(B, D, R1, R2, 1 if model is not None else 0)
+    if work is None:
+        work = torch.empty(need, dtype=torch.uint8, device='cuda')
+    lhat = torch.empty(B, D, dtype=torch.int8, device='cuda')
+    lt = ltrue.to(device='cuda', dtype=torch.int8).contiguous() if ltrue is not None else None
+    be = torch.empty(B, dtype=torch.int32, device='cuda') if lt is not None else None
+    mw = None
+    if model is not None:
+        W = model._weights_nmajor()
+        P_ = model.params
+        p = lambda t: t.data_ptr()
+        mw = _lib.MlpWeights(p(W['W1']), p(P_['b1']), p(W['W2']), p(P_['b2']), p(W['W3']), p(P_['b3']), model.F,
+                             model.H, float(model.inv_temp))
+    nr = Mr // 2
+    P_ = _lib.ptr
+    _lib.check(L.arp_detect_adaptive(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, int(M), P_(T1t), L1, R1, P_(T2t), L2, R2,
+                                     qi, float(nr + c * nr ** 0.5), P_(inst), key & MASK, (key >> 32) & MASK,
+                                     ctypes.byref(mw) if mw is not None else None, 1 if gate else 0, P_(lt),
+                                     P_(lhat), P_(be), P_(work), work.numel(), _lib.stream_ptr()), 'detect_adaptive')
+    return lhat, be

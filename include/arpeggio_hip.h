@@ -177,6 +177,39 @@ int arp_decode_select_list(const float *H, const float *y, const int8_t *xg, con
 int arp_certify(const float *best_d, const float *N0, float N0_scalar, float thr_scale, int B, int32_t *list,
                 int32_t *count, void *stream);
 
+/* Warm-start MLP weights for the whole-chain entry point (bf16 as uint16 bit patterns, n-major as
+ * arp_mlp_start_fused takes them): F = the feature row (4 D + 1 rounded up to a multiple of 32), H = 256, N = 2 D. */
+typedef struct {
+  const uint16_t *W1t;
+  const float *b1;
+  const uint16_t *W2t;
+  const float *b2;
+  const uint16_t *W3t;
+  const float *b3;
+  int F, H;
+  float inv_temp;
+} arp_mlp_weights;
+
+/* Bytes of workspace arp_detect_adaptive needs for B frames of D coordinates, R1 stage-1 and R2 stage-2 anneals per
+ * frame (learned = 1 with the warm-start MLP); 0 for invalid sizes. */
+size_t arp_detect_workspace_bytes(int B, int D, int R1, int R2, int learned);
+
+/* The whole detection chain in one call (dDI K = 1, first accept), replacing the reference's host sequence
+ * fxdetect.detect (arpeggio/mimo/fxdetect.py:24-43: MMSE, mapping, to_fixed, kernel, levels, select) for the
+ * first-accept solver: front end -> [warm-start MLP (mlp != NULL) -> start thresholds, energy gate `gate`] -> R1
+ * anneals of L1 passes (schedule T1) -> decode/select -> if R2 > 0: certificate d <= N0 thr_scale -> R2 more anneals
+ * (streams R1..R1+R2-1) of L2 passes (T2) on the open frames -> decode against stage 1's choice. R2 = 0 is the one-stage
+ * detector. H (B, Mr, D), y (B, Mr) float32; N0 (B) or NULL (N0_scalar); inst (B) Philox instance words; k0, k1 the
+ * key; q pinning in Q units; ltrue (B, D) int8 or NULL; outputs lhat (B, D) int8 and biterr (B) int32 (needs
+ * ltrue; may be NULL). work: device memory of at least arp_detect_workspace_bytes(B, D, R1, R2, mlp != NULL) bytes,
+ * owned by the caller; stage buffers are carved from it (256-byte aligned), so two calls that overlap on one stream
+ * order must not share it. Asynchronous on `stream`, graph-capturable (no allocation, no memset node, no host sync). */
+int arp_detect_adaptive(const float *H, const float *y, const float *N0, float N0_scalar, int B, int Mr, int D, int M,
+                        const int32_t *T1, int L1, int R1, const int32_t *T2, int L2, int R2, int q, float thr_scale,
+                        const uint32_t *inst, uint32_t k0, uint32_t k1, const arp_mlp_weights *mlp, int gate,
+                        const int8_t *ltrue, int8_t *lhat, int32_t *biterr, void *work, size_t work_bytes,
+                        void *stream);
+
 /* Learned-start MLP kernels (no reference counterpart). bf16 tensors are passed as their uint16 bit patterns.
  * arp_mlp_gemm: C(m, n) = sum_k A(m, k) B(k, n) (+ bias[n]), act 0 none / 1 GELU (tanh form), on v_mfma_f32_16x16x32_bf16
  *   with fp32 accumulation; A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]; outputs (each may be NULL): Cf

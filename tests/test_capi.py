@@ -22,7 +22,7 @@ def test_header_symbols_exported():
 
 def test_python_binding_covers_header():
     from arpeggio_amd import _lib
-    names = set(declared()) - {'arp_last_error', 'arp_abi_version'}
+    names = set(declared()) - {'arp_last_error', 'arp_abi_version'} - _lib.SPECIAL
     assert names <= set(_lib.SIGNATURES), names - set(_lib.SIGNATURES)
     L = _lib.load()
     assert L.arp_abi_version() >= 1
@@ -79,3 +79,22 @@ def test_oversized_workgroup_refused_without_gpu():
         assert rc == -1 and b'waves = 17' in L.arp_last_error()
     finally:
         assert L.arp_set_anneal_waves(0) == 0
+
+
+def test_detect_workspace_and_argument_errors_without_gpu():
+    """The whole-chain entry point sizes its workspace on the host and refuses a short one before any HIP call."""
+    import ctypes as ct
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    n = L.arp_detect_workspace_bytes(1000, 64, 4, 16, 1)
+    assert n >= 1000 * (64 * 64 * 4 + 20 * 128) and n % 256 == 0
+    assert L.arp_detect_workspace_bytes(1000, 64, 4, 0, 0) < n
+    assert L.arp_detect_workspace_bytes(10, 200, 4, 16, 0) == 0
+    buf = ct.create_string_buffer(64)
+    p = ct.cast(buf, ct.c_void_p)
+    rc = L.arp_detect_adaptive(p, p, None, 0.1, 1000, 64, 64, 16, p, 8, 4, p, 8, 16, 0, 40.0, p, 0, 0, None, 1, None, p,
+                               None, p, 64, None)
+    assert rc == -3 and b'workspace' in L.arp_last_error()
+    rc = L.arp_detect_adaptive(p, p, None, 0.1, 10, 64, 64, 8, p, 8, 4, p, 8, 16, 0, 40.0, p, 0, 0, None, 1, None, p,
+                               None, p, 1 << 30, None)
+    assert rc == -1 and b'M must be' in L.arp_last_error()

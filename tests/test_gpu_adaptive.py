@@ -120,3 +120,24 @@ def test_adaptive_graph_replay(model):
             assert torch.equal(a.cpu(), r.cpu()), (k, name)
         pipe.run(b['H'], b['y'], b['N0'], lt)
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize('start,R2', [('learned', 8), ('rand', 6), ('rand', 0), ('learned', 0)])
+def test_chain_entry_point_equals_pipelines(model, start, R2):
+    """arp_detect_adaptive (the whole chain in one C call, stage buffers carved from one workspace) gives the Python
+    pipelines' results bit for bit: AdaptivePipeline for R2 > 0, DetectPipeline for R2 = 0."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.detector import AdaptivePipeline, DetectPipeline, detect_chain
+    say('chain entry', start, R2)
+    B, key = 320, 0x2468
+    b = frames.draw(810, 21.0, 0, B, 32, 16, dev='cuda')
+    lt = b['l'].to(torch.int8).contiguous()
+    m = model if start == 'learned' else None
+    lh, be = detect_chain(b['H'], b['y'], b['N0'], 16, R1=4, L1=16, R2=R2, L2=32, c=2.0, model=m, key=key, ltrue=lt)
+    if R2 > 0:
+        pipe = AdaptivePipeline(B, 64, 64, 16, R1=4, L1=16, R2=R2, L2=32, c=2.0, start=start, key=key, model=m)
+    else:
+        pipe = DetectPipeline(B, 64, 64, 16, 4, cycles=16, start=start, key=key, model=m)
+    lh2, be2 = pipe.run(b['H'], b['y'], b['N0'], lt)
+    torch.cuda.synchronize()
+    assert torch.equal(lh, lh2) and torch.equal(be, be2)
