@@ -352,9 +352,10 @@ class AdaptivePipeline:
 
 
 def detect_chain(H, y, N0, M, R1=4, L1=24, R2=16, L2=64, c=3.0, model=None, gate=True, key=0, inst0=0, T0=0.1,
-                 T1=0.1, q=0.0, ltrue=None, Q=QSCALE, work=None):
+                 T1=0.1, q=0.0, ltrue=None, Q=QSCALE, work=None, noalloc=False):
     """The whole chain through the single C entry point arp_detect_adaptive (R2 = 0: one stage). Same results as
-    AdaptivePipeline (DetectPipeline for R2 = 0) on the same arguments. Returns (lhat, biterr or None)."""
+    AdaptivePipeline (DetectPipeline for R2 = 0) on the same arguments. noalloc: arp_detect_adaptive_noalloc, whose
+    stage buffers are the library's own device array (no workspace). Returns (lhat, biterr or None)."""
     H, y = _f32(H), _f32(y)
     B, Mr, D = H.shape
     L = _lib.load()
@@ -369,7 +370,7 @@ def detect_chain(H, y, N0, M, R1=4, L1=24, R2=16, L2=64, c=3.0, model=None, gate
     inst = ((torch.arange(B, dtype=torch.int64) + inst0) & MASK).to(torch.int32).to('cuda')
     n0t, n0s = _n0_arg(N0, B)
     need = L.arp_detect_workspace_bytes(B, D, R1, R2, 1 if model is not None else 0)
-    if work is None:
+    if work is None and not noalloc:
         work = torch.empty(need, dtype=torch.uint8, device='cuda')
     lhat = torch.empty(B, D, dtype=torch.int8, device='cuda')
     lt = ltrue.to(device='cuda', dtype=torch.int8).contiguous() if ltrue is not None else None
@@ -383,6 +384,13 @@ def detect_chain(H, y, N0, M, R1=4, L1=24, R2=16, L2=64, c=3.0, model=None, gate
                              model.H, float(model.inv_temp))
     nr = Mr // 2
     P_ = _lib.ptr
+    if noalloc:
+        _lib.check(L.arp_detect_adaptive_noalloc(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, int(M), P_(T1t), L1, R1, P_(T2t),
+                                                 L2, R2, qi, float(nr + c * nr ** 0.5), P_(inst), key & MASK,
+                                                 (key >> 32) & MASK, ctypes.byref(mw) if mw is not None else None,
+                                                 1 if gate else 0, P_(lt), P_(lhat), P_(be), _lib.stream_ptr()),
+                   'detect_adaptive_noalloc')
+        return lhat, be
     _lib.check(L.arp_detect_adaptive(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, int(M), P_(T1t), L1, R1, P_(T2t), L2, R2,
                                      qi, float(nr + c * nr ** 0.5), P_(inst), key & MASK, (key >> 32) & MASK,
                                      ctypes.byref(mw) if mw is not None else None, 1 if gate else 0, P_(lt),

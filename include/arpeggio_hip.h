@@ -210,6 +210,18 @@ int arp_detect_adaptive(const float *H, const float *y, const float *N0, float N
                         const int8_t *ltrue, int8_t *lhat, int32_t *biterr, void *work, size_t work_bytes,
                         void *stream);
 
+/* arp_detect_adaptive with no workspace argument and no allocation: the stage buffers live in a device array of the
+ * library (32 MiB per device, taken with hipGetSymbolAddress on the first call on each device). No reference
+ * counterpart: for callers that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation). A shape whose arp_detect_workspace_bytes exceeds 32 MiB is refused with ARP_E_RANGE (about
+ * 1,500 frames of 32 users at R1 + R2 = 20 fit). Every call on a device shares its array: calls that overlap on one
+ * device (other streams, other host threads) race and must be serialised by the caller. */
+int arp_detect_adaptive_noalloc(const float *H, const float *y, const float *N0, float N0_scalar, int B, int Mr, int D,
+                                int M, const int32_t *T1, int L1, int R1, const int32_t *T2, int L2, int R2, int q,
+                                float thr_scale, const uint32_t *inst, uint32_t k0, uint32_t k1,
+                                const arp_mlp_weights *mlp, int gate, const int8_t *ltrue, int8_t *lhat,
+                                int32_t *biterr, void *stream);
+
 /* Learned-start MLP kernels (no reference counterpart). bf16 tensors are passed as their uint16 bit patterns.
  * arp_mlp_gemm: C(m, n) = sum_k A(m, k) B(k, n) (+ bias[n]), act 0 none / 1 GELU (tanh form), on v_mfma_f32_16x16x32_bf16
  *   with fp32 accumulation; A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]; outputs (each may be NULL): Cf
