@@ -8,6 +8,7 @@
 #include "kernels.h"
 #include "launch_bounds.h"
 #include "adaptive.h"
+#include "mdi.h"
 
 namespace arp {
 
@@ -389,4 +390,37 @@ extern "C" int arp_certify(const float* best_d, const float* N0, float N0_scalar
   if (!list || !count || (B > 0 && !best_d)) return fail(ARP_E_ARG, "certify: null pointer");
   return hip_check(arp::launch_certify(best_d, N0, N0_scalar, thr_scale, B, list, count, (hipStream_t)stream),
                    "certify launch");
+}
+
+extern "C" int arp_mmse_sic(const float* H, const float* y, const float* N0, float N0_scalar, int B, int Mr, int D, int M,
+                            int per_user, int8_t* xs, void* stream) {
+  if (D < 2 || D > 128 || D % 2 || Mr < 1 || Mr > 128)
+    return fail(ARP_E_RANGE, "mmse_sic: D = %d (even, 2..128), Mr = %d (1..128)", D, Mr);
+  if (M != 4 && M != 16) return fail(ARP_E_ARG, "mmse_sic: M = %d", M);
+  if (B < 0) return fail(ARP_E_ARG, "mmse_sic: B = %d < 0", B);
+  if (B == 0) return ok();
+  if (!H || !y || !xs) return fail(ARP_E_ARG, "mmse_sic: null pointer");
+  return hip_check(arp::launch_mmse_sic(H, y, N0, N0_scalar, B, Mr, D, M, per_user ? 1 : 0, xs, (hipStream_t)stream),
+                   "mmse_sic launch");
+}
+
+extern "C" int arp_mdi_radius(const float* H, const float* y, const int8_t* xg, int B, int Mr, int D, int M, int Rmax,
+                              int32_t* R, float* Gam, float* dist, void* stream) {
+  if (D < 1 || D > 256 || Mr < 1 || Mr > 256) return fail(ARP_E_RANGE, "mdi_radius: D = %d, Mr = %d (1..256)", D, Mr);
+  if (M != 4 && M != 16) return fail(ARP_E_ARG, "mdi_radius: M = %d", M);
+  if (B < 0 || Rmax < 0) return fail(ARP_E_ARG, "mdi_radius: B = %d, Rmax = %d", B, Rmax);
+  if (B == 0) return ok();
+  if (!H || !y || !xg) return fail(ARP_E_ARG, "mdi_radius: null pointer");
+  return hip_check(arp::launch_mdi_radius(H, y, xg, B, Mr, D, M, Rmax, R, Gam, dist, (hipStream_t)stream),
+                   "mdi_radius launch");
+}
+
+extern "C" int arp_mdi_build(const float* H, const float* y, const int8_t* xg, const int32_t* list, int n, int Mr,
+                             int D, float Q, int32_t* Gi, int32_t* hi, void* stream) {
+  if (D < 1 || D > 128 || Mr < 1 || Mr > 128) return fail(ARP_E_RANGE, "mdi_build: D = %d, Mr = %d (1..128)", D, Mr);
+  if (n < 0) return fail(ARP_E_ARG, "mdi_build: n = %d < 0", n);
+  if (n == 0) return ok();
+  if (!H || !y || !xg || !list || !Gi || !hi) return fail(ARP_E_ARG, "mdi_build: null pointer");
+  return hip_check(arp::launch_mdi_build(H, y, xg, list, n, Mr, D, Q, Gi, hi, (hipStream_t)stream),
+                   "mdi_build launch");
 }

@@ -63,7 +63,8 @@ def decode_select(H, y, xg, spins, M, P=2, delta=True, ltrue=None, want_dist=Fal
     dist = torch.empty(B, R + 1, dtype=torch.float32, device=H.device) if want_dist else None
     lt = ltrue.to(device='cuda', dtype=torch.int8).contiguous() if ltrue is not None else None
     L = _lib.load()
-    _lib.check(L.arp_decode_select(_lib.ptr(H), _lib.ptr(y), _lib.ptr(xg.contiguous()), _lib.ptr(spins.contiguous()),
+    xg, spins = xg.contiguous(), spins.contiguous()        # held until the launch is queued behind their copies
+    _lib.check(L.arp_decode_select(_lib.ptr(H), _lib.ptr(y), _lib.ptr(xg), _lib.ptr(spins),
                                    B, R, P, D, Mr, int(M), 1 if delta else 0, _lib.ptr(lt), _lib.ptr(lhat),
                                    _lib.ptr(be), _lib.ptr(dist), _lib.stream_ptr()), 'decode_select')
     return lhat, be, dist

@@ -222,6 +222,26 @@ int arp_detect_adaptive_noalloc(const float *H, const float *y, const float *N0,
                                 const arp_mlp_weights *mlp, int gate, const int8_t *ltrue, int8_t *lhat,
                                 int32_t *biterr, void *stream);
 
+/* ---- Multi-stage delta-Ising (MDI) wrapper kernels (the host groups the sub-problems by radius, as the reference's
+ * mimo.mdi, arpeggio/mimo/mimo.py:546-602, and run_a513.mdi_logged, arpeggio/sim/local_code_a513/run_a513.py:43-80). */
+
+/* Ordered MMSE-SIC guess (arpeggio/mimo/mimo.py:466-503): per stage the stream (per_user = 0) or user (per_user = 1)
+ * with the smallest MMSE error variance, its bias-corrected estimate quantised and cancelled. H (B, Mr, D), y (B, Mr)
+ * float32, N0 (B) or NULL (N0_scalar); xs (B, D) int8 levels. D even, <= 128; Mr <= 128. */
+int arp_mmse_sic(const float *H, const float *y, const float *N0, float N0_scalar, int B, int Mr, int D, int M,
+                 int per_user, int8_t *xs, void *stream);
+
+/* Search radius of a guess (arpeggio/mimo/mimo.py:526-543, 'mean' heuristic) and the guess's ||y - H x_g||^2:
+ * R (B) int32 in [0, Rmax], Gam (B) float32, dist (B) float32 (each may be NULL). */
+int arp_mdi_radius(const float *H, const float *y, const int8_t *xg, int B, int Mr, int D, int M, int Rmax, int32_t *R,
+                   float *Gam, float *dist, void *stream);
+
+/* Fixed-point dDI sub-problems around the guesses xg (B, D) of the n frames list[0..n): Gi (B, D, D) and hi (B, D) int32
+ * written at each listed frame's index, equal to mimo.ising_delta_at -> chordfx.to_fixed -> chord_ref.compress
+ * (mimo.py:506-523, chordfx.py:60-76, chord_ref.py:47-59) up to float32 rounding; Q the fixed-point scale (4096). */
+int arp_mdi_build(const float *H, const float *y, const int8_t *xg, const int32_t *list, int n, int Mr, int D, float Q,
+                  int32_t *Gi, int32_t *hi, void *stream);
+
 /* Learned-start MLP kernels (no reference counterpart). bf16 tensors are passed as their uint16 bit patterns.
  * arp_mlp_gemm: C(m, n) = sum_k A(m, k) B(k, n) (+ bias[n]), act 0 none / 1 GELU (tanh form), on v_mfma_f32_16x16x32_bf16
  *   with fp32 accumulation; A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]; outputs (each may be NULL): Cf

@@ -1,0 +1,154 @@
+"""GPU parity of the MDI wrapper: the SIC guess, the search radius and the sub-problem build against the oracle, the
+whole chain against the oracle's chain on the same batch, and against the reference's own per-use fixtures
+(data/results/a513_fixed: its fixed-point Arpeggio inside MDI on the B200, seed 102 at 22 dB for 32 users).
+
+Exact equality is not the bar for the float stages: the per-stream SIC order between the two streams of a user is
+decided by rounding (their error variances are equal in exact arithmetic), and radius floors and distance comparisons
+can flip at float32 rounding; a different group structure also shifts the per-call keys of later sub-problems. The
+bars are per-frame agreement rates, exactness of the integer sub-problems away from rounding boundaries, and the
+paired bootstrap of per-use bit errors against the reference's fixtures."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, 'tests', 'golden')
+
+
+def say(*a):
+    print('[case]', *a, flush=True)
+
+
+@pytest.fixture(scope='module')
+def mdo(orc):
+    import mdi_oracle
+    return mdi_oracle
+
+
+@pytest.mark.parametrize('nt,M,snr,per_user', [(16, 16, 18.0, 0), (32, 16, 22.0, 0), (64, 16, 22.0, 0),
+                                                (16, 4, 8.0, 0), (32, 16, 20.0, 1)])
+def test_mmse_sic_vs_oracle(orc, mdo, nt, M, snr, per_user):
+    from arpeggio_amd import _lib, frames
+    say('sic', nt, M, snr, per_user)
+    B = 48
+    b = frames.draw(931, snr, 0, B, nt, M, dev='cuda')
+    xs = torch.empty(B, 2 * nt, dtype=torch.int8, device='cuda')
+    _lib.check(_lib.load().arp_mmse_sic(_lib.ptr(b['H']), _lib.ptr(b['y']), None, b['N0'], B, 2 * nt, 2 * nt, M,
+                                        per_user, _lib.ptr(xs), _lib.stream_ptr()), 'mmse_sic')
+    torch.cuda.synchronize()
+    got = xs.cpu().numpy()
+    H, y = b['H'].cpu().numpy(), b['y'].cpu().numpy()
+    per = 'complex' if per_user else 'real'
+    ok = np.zeros(B, dtype=bool)
+    for tie in (None, 'first', 'last'):
+        ok |= (mdo.mmse_sic(H, y, b['N0'], M, per=per, tie=tie) == got).all(1)
+    say('frames equal to an oracle order', int(ok.sum()), 'of', B)
+    assert ok.mean() >= (1.0 if per_user else 0.9)
+
+
+def test_mdi_radius_vs_oracle(orc, mdo):
+    from arpeggio_amd import _lib, frames
+    for nt, M, snr in ((16, 16, 18.0), (32, 16, 22.0), (64, 16, 22.0), (32, 4, 10.0)):
+        say('radius', nt, M, snr)
+        B = 64
+        b = frames.draw(932, snr, 0, B, nt, M, dev='cuda')
+        H, y = b['H'].cpu().numpy(), b['y'].cpu().numpy()
+        xg = mdo.mmse_guess(H, y, b['N0'], M)
+        xt = torch.as_tensor(xg.astype(np.int8)).cuda()
+        R = torch.empty(B, dtype=torch.int32, device='cuda')
+        G = torch.empty(B, device='cuda')
+        d = torch.empty(B, device='cuda')
+        _lib.check(_lib.load().arp_mdi_radius(_lib.ptr(b['H']), _lib.ptr(b['y']), _lib.ptr(xt), B, 2 * nt, 2 * nt, M,
+                                              3, _lib.ptr(R), _lib.ptr(G), _lib.ptr(d), _lib.stream_ptr()),
+                   'mdi_radius')
+        torch.cuda.synchronize()
+        Ro, Go = mdo.mdi_radius(H, y, xg, M, 3)
+        np.testing.assert_allclose(G.cpu().numpy(), Go, rtol=2e-5)
+        near = np.abs(Go / 2 - np.round(Go / 2)) < 1e-4
+        assert ((R.cpu().numpy() == Ro) | near).all()
+        np.testing.assert_allclose(d.cpu().numpy(), orc.dist(H, y, xg[:, None, :])[:, 0], rtol=2e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize('nt', [16, 32, 64])
+def test_mdi_build_vs_oracle(orc, mdo, nt):
+    """Sub-problems around a guess for a frame list: exact integers except within 0.05 of a rounding boundary; frames
+    off the list untouched."""
+    from arpeggio_amd import _lib, frames
+    say('build', nt)
+    B, D, M = 40, 2 * nt, 16
+    b = frames.draw(933, 18.0, 0, B, nt, M, dev='cuda')
+    H, y = b['H'].cpu().numpy(), b['y'].cpu().numpy()
+    rs = np.random.default_rng(nt)
+    xg = rs.choice([-3, -1, 1, 3], size=(B, D)).astype(np.int8)
+    idx = np.sort(rs.choice(B, size=25, replace=False)).astype(np.int32)
+    Gi = torch.full((B, D, D), 7, dtype=torch.int32, device='cuda')
+    hi = torch.full((B, D), 7, dtype=torch.int32, device='cuda')
+    xt, it = torch.as_tensor(xg).cuda(), torch.as_tensor(idx).cuda()   # held until the kernel has run
+    _lib.check(_lib.load().arp_mdi_build(_lib.ptr(b['H']), _lib.ptr(b['y']), _lib.ptr(xt), _lib.ptr(it), len(idx), D,
+                                         D, 4096.0, _lib.ptr(Gi), _lib.ptr(hi), _lib.stream_ptr()), 'mdi_build')
+    torch.cuda.synchronize()
+    Gg, hg = Gi.cpu().numpy(), hi.cpu().numpy()
+    Go, ho, scale = mdo.ddi_fixed(H[idx], y[idx], xg[idx].astype(np.float64), 1)
+    Gq = (np.swapaxes(H[idx], 1, 2).astype(np.float64) @ H[idx]) / scale[:, None, None] * 4096
+    hq = 2 * np.einsum('bij,bi->bj', H[idx].astype(np.float64),
+                       y[idx] - np.einsum('bij,bj->bi', H[idx].astype(np.float64), xg[idx])) / scale[:, None] * 4096
+    near_g = np.abs(np.abs(Gq - np.floor(Gq)) - 0.5) < 0.05
+    near_h = np.abs(np.abs(hq - np.floor(hq)) - 0.5) < 0.05
+    assert ((Gg[idx] == Go) | near_g).all() and ((hg[idx] == ho) | near_h).all()
+    off = np.setdiff1d(np.arange(B), idx)
+    assert (Gg[off] == 7).all() and (hg[off] == 7).all()
+
+
+def test_mdi_chain_vs_oracle(orc, mdo):
+    """The GPU MDI chain and the oracle's (numpy float64 + C fixed-point Arpeggio, the same keys and loops) on one 32x32
+    batch of the reference's run (seed 102, 22 dB, abl_far_32_0.1_0.1_0_rand_8): per-use decisions agree on >= 99 %."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.mdi import MDIDetector
+    rec = json.load(open(os.path.join(GOLDEN, 'a513_fixed.json')))['x32']
+    B = 400
+    b = frames.draw(rec['seed'], rec['snr'], 0, rec['m'], 32, 16, dev='cuda')
+    b = {k: (v[:B] if torch.is_tensor(v) else v) for k, v in b.items()}
+    ds = frames.draw_seed(rec['seed'], rec['snr'], 0)
+    det = MDIDetector(B, 32, 16, rec['pick'])
+    lh, ann, be = det.detect(b['H'], b['y'], b['N0'], ds, ltrue=b['l'])
+    torch.cuda.synchronize()
+    sol = mdo.ArpeggioMDISolver(ds, 32, 0.1, 0.1, 0.0, 'rand')
+    lo, anno = mdo.mdi(b['H'].cpu().numpy(), b['y'].cpu().numpy(), b['N0'], 16, sol, 8, 3, 8, ('mmse', 'sic'))
+    agree = (lh.cpu().numpy() == lo).all(1)
+    say('per-use agreement', agree.mean(), 'anneals equal', (ann == anno).mean())
+    assert agree.mean() >= 0.99
+    assert (ann == anno).mean() >= 0.9
+
+
+def test_mdi_vs_reference_fixtures(mdo):
+    """The GPU MDI chain on the reference's own batches (the same draws) against its per-use bit errors: paired
+    cluster bootstrap (arpeggio/sim/a56an.py:74-95) of our errors over theirs, whose 95 % interval must hold 1; per-use
+    bit errors equal on >= 99 % of uses; anneals per K within 3 % of the reference's counters."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.mdi import MDIDetector
+    from arpeggio_amd.stats import boot_ratio
+    z = np.load(os.path.join(GOLDEN, 'a513_fixed.npz'), allow_pickle=False)
+    rec = json.load(open(os.path.join(GOLDEN, 'a513_fixed.json')))['x32']
+    nb = 40
+    det = MDIDetector(rec['m'], 32, 16, rec['pick'])
+    ours, theirs = [], []
+    for bi in range(nb):
+        b = frames.draw(rec['seed'], rec['snr'], bi, rec['m'], 32, 16, dev='cuda')
+        _, ann, be = det.detect(b['H'], b['y'], b['N0'], frames.draw_seed(rec['seed'], rec['snr'], bi), ltrue=b['l'])
+        ours.append(be.cpu().numpy())
+        theirs.append(z['x32/biterr'][bi].astype(np.int64))
+    ours, theirs = np.concatenate(ours), np.concatenate(theirs)
+    r, lo, hi = boot_ratio(ours, theirs)
+    cnt_ref = z['x32/cnt'][:nb].sum(0)
+    say('BER ours %.3e reference %.3e ratio %.3f [%.3f, %.3f]' % (ours.sum() / ours.size / 64,
+                                                                   theirs.sum() / theirs.size / 64, r, lo, hi))
+    say('per-use equal', (ours == theirs).mean(), 'anneals per K', {K: det.cnt[K][2] for K in det.cnt},
+        'reference', cnt_ref[:, 2].tolist())
+    assert lo <= 1.0 <= hi
+    assert (ours == theirs).mean() >= 0.99
+    for K in (1, 2, 3):
+        assert abs(det.cnt[K][2] - cnt_ref[K - 1, 2]) <= 0.03 * cnt_ref[K - 1, 2]
