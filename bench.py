@@ -72,6 +72,9 @@ def parse():
     ap.add_argument('--no-64u', action='store_true', help='skip the 64-user correlated-channel measurement')
     ap.add_argument('--frames-64u', type=int, default=1 << 16)
     ap.add_argument('--copy-probe', action='store_true', help='run only a plain device copy of known size (PMC scale)')
+    ap.add_argument('--no-mdi', action='store_true', help='skip the MDI wrapper against the reference fixtures')
+    ap.add_argument('--mdi-batches', type=int, default=60, help='32-user fixture batches (1,000 frames each)')
+    ap.add_argument('--mdi-large', type=int, default=16384, help='frames of the MDI throughput batch')
     return ap.parse_args()
 
 
@@ -316,6 +319,65 @@ def feat_len():
     return (4 * 2 * NT + 1 + 31) // 32 * 32
 
 
+def mdi(args):
+    """The MDI wrapper with the fixed-point first-accept solver on the reference's own batches (its draws; seeds, SNRs
+    and picks of data/results/a513_fixed, copied into tests/golden/a513_fixed.npz): per-use bit errors paired with the
+    reference's (bootstrap CI of the BER ratio), anneals per K against its counters, and detections/s (GPU time of the
+    detect calls, host grouping included; the draws are not timed). Then the throughput of one large batch."""
+    import torch
+    from arpeggio_amd import frames
+    from arpeggio_amd.mdi import MDIDetector
+    from arpeggio_amd.stats import boot_ratio
+    gd = os.path.join(ROOT, 'tests', 'golden')
+    z = np.load(os.path.join(gd, 'a513_fixed.npz'), allow_pickle=False)
+    idx = json.load(open(os.path.join(gd, 'a513_fixed.json')))
+    out = {}
+    for tag, nb in (('x32', args.mdi_batches), ('x16', max(1, args.mdi_batches // 5)),
+                    ('x64', max(1, args.mdi_batches // 2))):
+        rec = idx[tag]
+        nb = min(nb, rec['batches'])
+        det = MDIDetector(rec['m'], rec['nt'], 16, rec['pick'])
+        ours, wall = [], 0.0
+        for bi in range(nb):
+            b = frames.draw(rec['seed'], rec['snr'], bi, rec['m'], rec['nt'], 16, dev='cuda')
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, be = det.detect(b['H'], b['y'], b['N0'], frames.draw_seed(rec['seed'], rec['snr'], bi), ltrue=b['l'])
+            torch.cuda.synchronize()
+            wall += time.perf_counter() - t0
+            ours.append(be.cpu().numpy().astype(np.int64))
+        ours = np.concatenate(ours)
+        theirs = z['%s/biterr' % tag][:nb].reshape(-1).astype(np.int64)
+        r, lo, hi = boot_ratio(ours, theirs)
+        nbits = ours.size * 4 * rec['nt']
+        cref = z['%s/cnt' % tag][:nb].sum(0)
+        out[tag] = dict(users=rec['nt'], snr_db=rec['snr'], pick=rec['pick'], seed=rec['seed'], batches=nb,
+                        uses=int(ours.size), ber=float(ours.sum()) / nbits, reference_ber=float(theirs.sum()) / nbits,
+                        ratio_to_reference=r, ci=[lo, hi], per_use_equal=float((ours == theirs).mean()),
+                        anneals_per_K=[int(det.cnt.get(K, [0, 0, 0])[2]) for K in (1, 2, 3)],
+                        reference_anneals_per_K=cref[:, 2].tolist(),
+                        flips_per_K=[int(det.cnt.get(K, [0, 0, 0])[0]) for K in (1, 2, 3)],
+                        reference_flips_per_K=cref[:, 0].tolist(), value=ours.size / wall, unit='detections/s',
+                        batch=rec['m'])
+        log('MDI %s: BER %.3e vs reference %.3e, ratio %.3f [%.3f, %.3f], %.0f detections/s' % (
+            tag, out[tag]['ber'], out[tag]['reference_ber'], r, lo, hi, out[tag]['value']))
+    # one large batch (throughput; the per-batch keys of a 65,536-frame batch have no reference counterpart)
+    rec = idx['x32']
+    Bl = args.mdi_large
+    b = frames.draw(rec['seed'], rec['snr'], 9000, Bl, 32, 16, dev='cuda')
+    det = MDIDetector(Bl, 32, 16, rec['pick'])
+    det.detect(b['H'], b['y'], b['N0'], 1, ltrue=b['l'])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, be = det.detect(b['H'], b['y'], b['N0'], 2, ltrue=b['l'])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out['x32_large'] = dict(frames=Bl, value=Bl / dt, unit='detections/s', ms_per_batch=dt * 1e3,
+                            ber=float(be.sum()) / (Bl * 128))
+    log('MDI x32, %d frames in one batch: %.0f detections/s' % (Bl, Bl / dt))
+    return out
+
+
 def batched(args):
     """Mixed-size batch: 2^17 frames, 40 / 40 / 20 % of 16 / 32 / 64 users, QPSK and 16-QAM half each, i.i.d.
     Rayleigh, per-frame SNR uniform in 8-24 dB (GPU generator), Arpeggio R = 8, 64 passes for every class. One step =
@@ -481,6 +543,9 @@ def main():
     if not args.no_cpu:
         log('cpu baseline (3 segments)')
         out['cpu_baseline'] = cpu_baseline(b, plain, args)
+    if not args.no_mdi:
+        log('MDI wrapper against the reference fixtures')
+        out['mdi'] = mdi(args)
     if not args.no_64u:
         log('64 users, correlated channel (%d frames)' % args.frames_64u)
         out['users64'] = users64(args)
