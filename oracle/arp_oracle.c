@@ -10,6 +10,10 @@
  *   - Arpeggio first-accept ('arpeggio') and parallel ('par') passes: arpeggio_ref59.py:41-85
  *     (GPU form with identical semantics: arpeggio/sim/arpeggio_fx.cu:35-100; host API arpeggio/sim/arpeggiofx.py:23-44)
  *   - Chord snapshot commit with copy phases, Onsager c_k and pinning q: arpeggio/mimo/chordfx.py:80-118
+ *   - the production engine's lane-shared Chord (one field per coordinate, P copies, every accepted lane of the active
+ *     copy committed from one snapshot) with its xorshift32 lane streams or Philox: arpeggio/sim/x3model/chord_ref.py:
+ *     17-45 (fmix32, xs32, lane_seeds) and 73-123 (chord_ref); pinned against the reference's own cycle model
+ *     (arpeggio/sim/x3model/engine_model.cpp, built into oracle/_ref by this directory's Makefile)
  *   - FP32/FP64 float twins of both anneals with Philox uniforms r = (U - 1/2) 4T (the reference's float solvers,
  *     arpeggio/sim/arpx55.py:29-99 rule 'r' and arpeggio/mimo/mimo.py:204-249, draw from torch generators; this
  *     oracle feeds them the fixed-point path's Philox words instead).
@@ -681,3 +685,130 @@ int orc_chord_fx(const int16_t *J, const int32_t *h, const int32_t *h0, int B, i
 
 FLOAT_TWINS(float, f32)
 FLOAT_TWINS(double, f64)
+
+/* ------------------------------------------------------------------ lane-shared Chord (chord_ref.chord_ref) */
+/*
+ * G (D x D, symmetric, with diagonal), h (D), P copies, S steps with T_k, c_k, pinning q; anneal r of instance inst,
+ * 64-bit seed. Streams: rng 1 = production xorshift32, one state per coordinate j seeded
+ * fmix32(fmix32(fmix32(fmix32(lo ^ inst) ^ hi) ^ r) + (j + 1) 0x9E3779B9) (0 -> 0x9E3779B9), stepped once before the
+ * start and once per step, start spin of copy p = bit 31 - p, step draw u = x >> 16 (RB = 16 fraction bits); rng 0 =
+ * Philox, start Philox(2^32 - 1, p D + j, r, inst)[0] >= 2^31, step draw Philox(n >> 2, p D + j, r, inst)[n & 3]
+ * with n = k div P (RB = 32). Step k: p = k mod P, A = 4 T_k, B_k = q - c_k + 2 T_k; lane j of copy p flips iff
+ * u A > (s_pj w_j + 2 G_jj + B_k) 2^RB; every flip of the step commits from the same fields: sf = s_p on flipped
+ * lanes, de += 4 sf.w + 8 sf^T G sf, w += 4 G sf. The state is kept when de < best (first minimum).
+ */
+static inline uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    return h ^ (h >> 16);
+}
+
+static inline uint32_t xs32(uint32_t x)
+{
+    x ^= x << 13;
+    x ^= x >> 17;
+    return x ^ (x << 5);
+}
+
+static void chord_lane_one(const int32_t *G, const int32_t *h, int D, int P, const int32_t *T, const int32_t *c, int S,
+                           int q, uint32_t r, uint32_t inst, uint64_t seed, int rng, int8_t *bs, int64_t *best_out,
+                           int64_t *flips_out)
+{
+    const uint32_t lo = (uint32_t)(seed & MASK32), hi = (uint32_t)(seed >> 32);
+    int8_t *s = (int8_t *)malloc((size_t)P * D);
+    uint32_t *x = (uint32_t *)malloc(sizeof(uint32_t) * D);
+    int64_t *m = (int64_t *)malloc(sizeof(int64_t) * D), *w = (int64_t *)malloc(sizeof(int64_t) * D);
+    int64_t *sf = (int64_t *)malloc(sizeof(int64_t) * D), *gs = (int64_t *)malloc(sizeof(int64_t) * D);
+    const uint32_t base = fmix32(fmix32(fmix32(lo ^ inst) ^ hi) ^ r);
+    for (int j = 0; j < D; j++) {
+        if (rng) {
+            uint32_t st = fmix32(base + (uint32_t)(j + 1) * 0x9E3779B9u);
+            x[j] = xs32(st ? st : 0x9E3779B9u);
+            for (int p = 0; p < P; p++) s[p * D + j] = ((x[j] >> (31 - p)) & 1u) ? 1 : -1;
+        } else {
+            for (int p = 0; p < P; p++)
+                s[p * D + j] = philox_word(MASK32, (uint32_t)(p * D + j), r, inst, lo, hi, 0) >= 0x80000000u ? 1 : -1;
+        }
+    }
+    fields_of(G, h, D, P, s, m, w);
+    int64_t de = 0, best = (int64_t)1 << 62, flips = 0;
+    memcpy(bs, s, (size_t)P * D);
+    const int RB = rng ? 16 : 32;
+    for (int k = 0; k < S; k++) {
+        const int p = k % P, n = k / P;
+        const int64_t A = 4 * (int64_t)T[k], Bk = (int64_t)q - c[k] + 2 * (int64_t)T[k];
+        for (int j = 0; j < D; j++) {
+            uint64_t u;
+            if (rng) {
+                x[j] = xs32(x[j]);
+                u = x[j] >> 16;
+            } else {
+                u = philox_word((uint32_t)(n >> 2), (uint32_t)(p * D + j), r, inst, lo, hi, n & 3);
+            }
+            const int64_t zp = s[p * D + j] * w[j] + 2 * (int64_t)G[(size_t)j * D + j] + Bk;
+            const __int128 lhs = (__int128)u * A, rhs = (__int128)zp * ((__int128)1 << RB);
+            sf[j] = lhs > rhs ? s[p * D + j] : 0;
+        }
+        int64_t d1 = 0, d2 = 0;
+        for (int i = 0; i < D; i++) {
+            int64_t acc = 0;
+            for (int j = 0; j < D; j++) acc += (int64_t)G[(size_t)i * D + j] * sf[j];
+            gs[i] = acc;
+            d1 += sf[i] * w[i];
+            d2 += sf[i] * acc;
+        }
+        de += 4 * d1 + 8 * d2;
+        for (int i = 0; i < D; i++) {
+            w[i] += 4 * gs[i];
+            if (sf[i]) {
+                s[p * D + i] = (int8_t)-s[p * D + i];
+                flips++;
+            }
+        }
+        if (de < best) {
+            best = de;
+            memcpy(bs, s, (size_t)P * D);
+        }
+    }
+    *best_out = best;
+    *flips_out = flips;
+    free(s);
+    free(x);
+    free(m);
+    free(w);
+    free(sf);
+    free(gs);
+}
+
+typedef struct {
+    const int32_t *G, *h, *T, *c;
+    const uint32_t *inst;
+    int D, P, R, S, q, rng;
+    uint64_t seed;
+    int8_t *out_s;
+    int64_t *out_de, *out_fl;
+} chord_lane_ctx;
+
+static void chord_lane_item(void *vctx, long t)
+{
+    chord_lane_ctx *c = (chord_lane_ctx *)vctx;
+    long b = t / c->R;
+    uint32_t r = (uint32_t)(t % c->R);
+    chord_lane_one(c->G + (size_t)b * c->D * c->D, c->h + (size_t)b * c->D, c->D, c->P, c->T, c->c, c->S, c->q, r,
+                   c->inst[b], c->seed, c->rng, c->out_s + (size_t)t * c->P * c->D, c->out_de + t, c->out_fl + t);
+}
+
+/* chord_ref.chord_ref on (B, D, D) / (B, D) integers: out_s (B, R, P D) copy-major best spins, out_de (B, R) best
+ * energy relative to the start, out_fl (B, R) flips. */
+int orc_chord_lane(const int32_t *G, const int32_t *h, int B, int D, int P, int R, const int32_t *T, const int32_t *c,
+                   int S, int q, const uint32_t *inst, uint64_t seed, int rng, int8_t *out_s, int64_t *out_de,
+                   int64_t *out_fl, int nthreads)
+{
+    if (D < 1 || S < 0 || R < 1 || B < 0 || P < 1 || P > 32) return -1;
+    chord_lane_ctx cx = {G, h, T, c, inst, D, P, R, S, q, rng, seed, out_s, out_de, out_fl};
+    run_pool(chord_lane_item, &cx, (long)B * R, nthreads);
+    return 0;
+}

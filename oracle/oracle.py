@@ -56,6 +56,7 @@ def lib():
                                          ctypes.c_long, P]
         L.orc_arpeggio_ws.argtypes = [P, P, i32, i32, i32, i32, P, i32, i32, P, u64, P, i32, P, P, P, P, i32]
         L.orc_chord_fx.argtypes = [P, P, P, i32, i32, P, P, i32, i32, i32, i32, u64, i64, i32, P, P, P, i32]
+        L.orc_chord_lane.argtypes = [P, P, i32, i32, i32, i32, P, P, i32, i32, P, u64, i32, P, P, P, i32]
         L.orc_arpeggio_float_f32.argtypes = [P, P, i32, i32, i32, i32, P, i32, ctypes.c_float, P, u64, i32, P, P, P,
                                              P, i32]
         L.orc_arpeggio_float_f64.argtypes = [P, P, i32, i32, i32, i32, P, i32, ctypes.c_double, P, u64, i32, P, P, P,
@@ -166,6 +167,63 @@ def chord_fx(F, R, seed, P=1, inst0=0, mode='chains', nthreads=None, stats=None)
     if stats is not None:
         stats[:] = st
     return s, e
+
+
+def chord_lane(G, h, P, R, T, c, q, inst, seed, rng='xs', nthreads=None):
+    """chord_ref.chord_ref (arpeggio/sim/x3model/chord_ref.py:73-123): the production engine's lane-shared Chord on
+    G (B, D, D), h (B, D) integers, schedule T, c (S), pinning q, instance words inst (B), 64-bit seed; rng 'xs'
+    (xorshift32 lane streams) or 'philox'. Returns spins (B, R, P D) int8, best de (B, R), flips (B, R)."""
+    G = np.ascontiguousarray(G, dtype=np.int32)
+    h = np.ascontiguousarray(h, dtype=np.int32)
+    B, D = h.shape
+    T = np.ascontiguousarray(T, dtype=np.int32)
+    c = np.ascontiguousarray(c, dtype=np.int32)
+    inst = np.ascontiguousarray(np.asarray(inst, dtype=np.int64) & MASK32, dtype=np.uint32)
+    s = np.zeros((B, R, P * D), dtype=np.int8)
+    de = np.zeros((B, R), dtype=np.int64)
+    fl = np.zeros((B, R), dtype=np.int64)
+    rc = lib().orc_chord_lane(_p(G), _p(h), B, D, P, R, _p(T), _p(c), len(T), int(q), _p(inst),
+                              seed & 0xFFFFFFFFFFFFFFFF, 1 if rng == 'xs' else 0, _p(s), _p(de), _p(fl),
+                              _threads(nthreads))
+    assert rc == 0
+    return s, de, fl
+
+
+REF_ENGINE = os.path.join(HERE, '_ref', 'libape.so')
+_eng = None
+
+
+def ref_engine():
+    """The reference's cycle-accurate Chord engine (arpeggio/sim/x3model/engine_model.cpp) built by `make -C oracle
+    ref` into oracle/_ref, or None where it is not built (the reference sources are present in the build container
+    only; the built library travels with the tree)."""
+    global _eng
+    if _eng is None and os.path.exists(REF_ENGINE):
+        L = ctypes.CDLL(REF_ENGINE)
+        L.engine_solve_batch.restype = ctypes.c_int
+        _eng = L
+    return _eng
+
+
+def ref_engine_solve(G, h, P, R, T, c, q, inst, seed, philox=0, A=4, SEEDL=None, nthreads=8):
+    """engine_solve_batch (engine_model.cpp:336-380) as the reference's engine_py.solve_batch calls it
+    (engine_py.py:116-133): spins (B, R, P D) int8, de (B, R), cycles (B,), stats (B, 4) = D-ops, F-ops, I-ops,
+    R-port busy cycles."""
+    L = ref_engine()
+    G = np.ascontiguousarray(G, dtype=np.int16)
+    h = np.ascontiguousarray(h, dtype=np.int16)
+    B, D, _ = G.shape
+    SEEDL = SEEDL or max(1, D // 32)
+    T = np.ascontiguousarray(T, dtype=np.int16)
+    c = np.ascontiguousarray(c, dtype=np.int16)
+    inst = np.ascontiguousarray(np.asarray(inst, dtype=np.int64) & MASK32, dtype=np.uint32)
+    spins = np.zeros((B, R, P * D), np.int8)
+    de = np.zeros((B, R), np.int64)
+    cyc = np.zeros(B, np.int64)
+    st = np.zeros((B, 4), np.int64)
+    L.engine_solve_batch(D, A, SEEDL, philox, B, _p(G), _p(h), P, R, len(T), _p(T), _p(c), int(q), _p(inst),
+                         ctypes.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), _p(spins), _p(de), _p(cyc), _p(st), nthreads)
+    return spins, de, cyc, st
 
 
 def arpeggio_float(G, h, P, R, T, q, inst, key, start, dtype=np.float32, nthreads=None):

@@ -282,3 +282,28 @@ def test_draws_match_reference(golden):
         lt = torch.as_tensor(z[p + 'l'])
         bits = frames.levels_to_bits(lt, f['M'])
         assert bits.shape[-1] == lt.shape[-1] * (2 if f['M'] == 16 else 1)
+
+
+def test_product_to_fixed_and_geom_golden(golden):
+    """The product's chordfx.to_fixed and geom (the drop-in module) equal the reference's on its golden inputs,
+    including .5 ties, the Onsager ramp and the schedules of the front-end cases."""
+    import torch
+    from arpeggio_amd import chordfx
+    z, idx = golden
+    t = idx['front']['tofixed']
+    I = dict(J=torch.as_tensor(z['tofixed/Jin']), h=torch.as_tensor(z['tofixed/hin']),
+             h0=torch.as_tensor(z['tofixed/hin']))
+    F = chordfx.to_fixed(I, t['S'], t['t0'], t['t1'], t['qf'], t['kappa'])
+    for k in ('J', 'h', 'T', 'c'):
+        assert (F[k].numpy() == z['tofixed/' + k]).all(), k
+    assert F['q'] == t['q']
+    assert np.array_equal(chordfx.geom(0.3, 0.2, 256).numpy(), z['geom/a'])
+    assert np.array_equal(chordfx.geom(1.2, 0.1, 7).numpy(), z['geom/b'])
+    for tag, f in idx['front'].items():
+        if tag == 'tofixed':
+            continue
+        p = 'front/%s/' % tag
+        F = chordfx.to_fixed(dict(J=torch.zeros(1, 2, 2), h=torch.zeros(1, 2), h0=torch.zeros(1, 2)), 64, 0.3, 0.2,
+                             0.4, 0.3)
+        assert (F['T'].numpy() == z[p + 'T']).all() and (F['c'].numpy() == z[p + 'c']).all(), tag
+        assert F['q'] == f['q']
