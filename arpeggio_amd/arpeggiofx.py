@@ -103,3 +103,39 @@ def solve_float_device(G, h, P, R, T, q, inst, key, start, want_fields=False, tr
                                          0 if start == 'rand' else 1, _lib.ptr(out_s), _lib.ptr(out_de),
                                          _lib.ptr(out_st), _lib.ptr(out_w), _lib.stream_ptr()), 'arpeggio_f32')
     return out_s, out_de, out_st, out_w
+
+
+def chord_ref_device(G, h, P, T, c, q, R, seed, inst0=0, rng='xs', inst=None, transposed=False):
+    """The production engine's lane-shared Chord on the GPU (arp_anneal_chord_lane), with chord_ref.chord_ref's
+    arguments (arpeggio/sim/x3model/chord_ref.py:73-123): G (B, D, D) / h (B, D) compressed integers, P copies,
+    schedule T and Onsager terms c (S each, c may be None), pinning q, R anneals, 64-bit seed, instance words inst0 + b
+    (or inst (B,)), rng 'xs' (xorshift32 lane streams, 16-bit draws) or 'philox'. Returns device tensors: best spins
+    (B, R, P D) int8, best de (B, R) int64 and (B, R, 2) int64 whose [..., 0] is each anneal's flip count (the
+    reference returns flips per step summed over anneals)."""
+    G = torch.as_tensor(G)
+    h = torch.as_tensor(h)
+    assert G.dim() == 3 and G.shape[1] == G.shape[2], 'G must be (B, D, D)'
+    B, D, _ = G.shape
+    assert h.shape == (B, D), 'h must be (B, D)'
+    if rng not in ('xs', 'philox'):
+        raise ValueError("rng must be 'xs' or 'philox'")
+    Tl = [int(x) for x in (T.tolist() if torch.is_tensor(T) else T)]
+    cl = [0] * len(Tl) if c is None else [int(x) for x in (c.tolist() if torch.is_tensor(c) else c)]
+    if len(cl) != len(Tl):
+        raise ValueError('chord_ref: %d Onsager terms for %d steps' % (len(cl), len(Tl)))
+    _range_check(G, h, P, Tl, abs(int(q)) + max(abs(x) for x in cl))
+    Gt = (G if transposed else G.transpose(1, 2)).to(device='cuda', dtype=torch.int32).contiguous()
+    hh = h.to(device='cuda', dtype=torch.int32).contiguous()
+    Tt = torch.tensor(Tl, dtype=torch.int32, device='cuda')
+    Ct = torch.tensor(cl, dtype=torch.int32, device='cuda')
+    iv = np.arange(B, dtype=np.int64) + int(inst0) if inst is None else np.asarray(
+        torch.as_tensor(inst).cpu().numpy(), dtype=np.int64)
+    it = torch.as_tensor(iv & M32).to(torch.int32).cuda()
+    out_s = torch.empty(B, R, P * D, dtype=torch.int8, device='cuda')
+    out_de = torch.empty(B, R, dtype=torch.int64, device='cuda')
+    out_st = torch.empty(B, R, 2, dtype=torch.int64, device='cuda')
+    _lib.check(_lib.load().arp_anneal_chord_lane(
+        _lib.ptr(Gt), _lib.ptr(hh), _lib.ptr(Tt), _lib.ptr(Ct), len(Tl), int(q), int(P), int(R), int(D), int(B),
+        _lib.ptr(it), seed & M32, (seed >> 32) & M32, 1 if rng == 'xs' else 0, _lib.ptr(out_s), _lib.ptr(out_de),
+        _lib.ptr(out_st), _lib.stream_ptr()), 'chord_lane')
+    return out_s, out_de, out_st

@@ -7,7 +7,8 @@
 namespace arp {
 
 // Optional arguments of arpeggio_fx_kernel: learned-start thresholds (B, P D) with the energy gate and the kept flags,
-// a frame list (slot i anneals frame list[i] for i < *count), and the first Philox stream index r0.
+// a frame list (slot i anneals frame list[i] for i < *count), the first stream index r0, an Onsager term C per pass
+// (subtracted from the decision bound), and rng 1 for the production xorshift lane streams (parallel pass only).
 struct AnnealExtra {
   const uint32_t* thr;
   int8_t* used;
@@ -15,6 +16,8 @@ struct AnnealExtra {
   const int32_t* count;
   int gate;
   int r0;
+  const int32_t* C;
+  int rng;
 };
 
 // Optional arguments of the decode kernels: a frame list (as AnnealExtra; spins are read from slot i, everything else

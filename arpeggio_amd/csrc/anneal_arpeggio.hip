@@ -115,20 +115,41 @@ __device__ __forceinline__ long long start_energy(const int32_t* __restrict__ hb
 
 // DS: D as a compile-time constant (0: D at run time); with it the column address l*D folds into the LDS address
 // shift and the lane masks into constants.
-template <int P, int CPL, int MODE, bool FULL, int DS = 0>
+// fmix32 (the MurmurHash3 finaliser) and xorshift32 (13, 17, 5): the production engine's stream primitives
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  return h ^ (h >> 16);
+}
+
+__device__ __forceinline__ uint32_t xs32(uint32_t x) {
+  x ^= x << 13;
+  x ^= x >> 17;
+  return x ^ (x << 5);
+}
+
+// RNG 1: the production engine's xorshift32 lane streams (one state per coordinate, seeded from the key, instance
+// and anneal by fmix32; start spin of copy p = bit 31 - p of the first state, each pass draws u = x >> 16 from the
+// next, RB = 16 fraction bits), as arpeggio/sim/x3model/chord_ref.py:17-45 and 99-107 define them. Cq (per pass, may be
+// NULL): an Onsager term subtracted from the decision bound, B_k = q - c_k + 2 T_k (chord_ref.py:102).
+template <int P, int CPL, int MODE, bool FULL, int DS = 0, int RNG = 0>
 __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb,
                                                 const int32_t* __restrict__ T, int L, int q, int D, uint32_t r,
                                                 uint32_t inst, uint32_t k0, uint32_t k1, int start,
                                                 int8_t* __restrict__ out_s, long long* __restrict__ out_de,
                                                 long long* __restrict__ out_st, size_t t,
                                                 const uint32_t* __restrict__ thr, int gate,
-                                                int8_t* __restrict__ used) {
+                                                int8_t* __restrict__ used, const int32_t* __restrict__ Cq = nullptr) {
   if (DS) D = DS;
   const int lane = lane_id();
   bool valid[CPL];
   int32_t w[CPL], dj2[CPL];
   uint64_t vmask[CPL];
   uint64_t Sm[P][CPL], Rm[P][CPL];  // start spins as wave masks; Rm: the random start beside a learned one
+  uint32_t xsr[CPL];                // RNG 1: this lane's xorshift state per column block
+  const uint32_t xs_base = RNG == 1 ? fmix32(fmix32(fmix32(k0 ^ inst) ^ k1) ^ r) : 0u;
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     const int j = c * ARP_WAVE + lane;
@@ -136,7 +157,12 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
     const int nv = D - c * ARP_WAVE;
     vmask[c] = nv >= ARP_WAVE ? ~0ull : (nv <= 0 ? 0ull : ((1ull << nv) - 1ull));
     dj2[c] = valid[c] ? G4[j * D + j] >> 1 : 0;
-    if (thr) {
+    if (RNG == 1) {
+      const uint32_t st = fmix32(xs_base + (uint32_t)(j + 1) * 0x9E3779B9u);
+      xsr[c] = xs32(st ? st : 0x9E3779B9u);
+#pragma unroll
+      for (int p = 0; p < P; ++p) Sm[p][c] = ballot(valid[c] && ((xsr[c] >> (31 - p)) & 1u));
+    } else if (thr) {
       // learned start: spin +1 iff x + thr_pj >= 2^32, x the random start's word; Rm holds the random start
 #pragma unroll
       for (int p = 0; p < P; ++p) {
@@ -199,16 +225,18 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
   uint32_t rw[P][CPL][4];
   for (int kb = 0; kb < L; kb += 4 * P) {
     const int n4 = kb / (4 * P);
-    int32_t Tb[4 * P];
+    int32_t Tb[4 * P], Cb[4 * P];
 #pragma unroll
     for (int i = 0; i < 4 * P; ++i) Tb[i] = (FULL || kb + i < L) ? T[kb + i] : 0;  // FULL: L % 4P == 0
+#pragma unroll
+    for (int i = 0; i < 4 * P; ++i) Cb[i] = (Cq && (FULL || kb + i < L)) ? Cq[kb + i] : 0;
 #pragma unroll
     for (int nm = 0; nm < 4; ++nm) {
 #pragma unroll
       for (int p = 0; p < P; ++p) {
         const int k = kb + nm * P + p;
         if (!FULL && k >= L) break;
-        if (nm == 0) {
+        if (RNG == 0 && nm == 0) {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             uint32_t c0 = (uint32_t)n4, c1 = (uint32_t)(p * D + c * ARP_WAVE + lane), c2 = r, c3 = inst;
@@ -224,10 +252,19 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
         // Decision x A > (q_j + B_k) 2^32 (B_k = q + 2 T_k, q_j = s w_j + 2 G_jj) as q_j <= tq with
         // tq = floor((x A - 1) / 2^32) - B_k: the high word of one 32x32+64 multiply-add with addend -1 (x A = 0
         // gives -1), then one subtraction; the documented ranges keep tq in int32.
-        const int32_t Bk = q + 2 * Tk;
+        const int32_t Bk = q + 2 * Tk - Cb[nm * P + p];
         int32_t tq[CPL];
+        if (RNG == 1) {
+          // u A > (q_j + B_k) 2^16 with the 16-bit draw u of the lane's next xorshift state
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) tq[c] = (int32_t)(((uint64_t)rw[p][c][nm] * A + ~0ull) >> 32) - Bk;
+          for (int c = 0; c < CPL; ++c) {
+            xsr[c] = xs32(xsr[c]);
+            tq[c] = (int32_t)(((uint64_t)(xsr[c] >> 16) * A + ~0ull) >> 16) - Bk;
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) tq[c] = (int32_t)(((uint64_t)rw[p][c][nm] * A + ~0ull) >> 32) - Bk;
+        }
         // Within a pass a lane flips at most once and is then behind the scan, so the lanes still to be tested keep
         // their start-of-pass signs sg and q = (w ^ sg) - sg + dj2.
         if (MODE == 0) {
@@ -334,7 +371,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
 // lanes past D of the last column stay inside the allocation). With a frame list, workgroup i anneals frame list[i]
 // if i < *count and exits otherwise (a uniform exit before any barrier); its outputs go to slot i. Anneal rr of a
 // workgroup uses Philox stream r0 + rr, so a later launch with r0 = R continues a frame's anneals with new streams.
-template <int P, int CPL, int MODE, bool FULL, int DS>
+template <int P, int CPL, int MODE, bool FULL, int DS, int RNG>
 __global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt,
                                                                                const int32_t* __restrict__ h,
                                                                                const int32_t* __restrict__ T, int L,
@@ -361,19 +398,20 @@ __global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) void arpeggio_fx_kernel(c
   const uint32_t in = inst[b];
   const uint32_t* thr = ex.thr ? ex.thr + (size_t)b * P * D : nullptr;
   for (int r = wv; r < R; r += W)
-    arpeggio_anneal<P, CPL, MODE, FULL, DS>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)(ex.r0 + r), in, k0, k1, start,
-                                            out_s, out_de, out_st, (size_t)slot * R + r, thr, ex.gate, ex.used);
+    arpeggio_anneal<P, CPL, MODE, FULL, DS, RNG>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)(ex.r0 + r), in, k0, k1,
+                                                 start, out_s, out_de, out_st, (size_t)slot * R + r, thr, ex.gate,
+                                                 ex.used, ex.C);
 }
 
-template <int P, int CPL, int MODE, bool FULL, int DS>
+template <int P, int CPL, int MODE, bool FULL, int DS, int RNG = 0>
 static void launch_pcmf(dim3 grid, dim3 block, size_t lds, hipStream_t st, const int32_t* Gt, const int32_t* h,
                         const int32_t* T, int L, int q, int R, int D, const uint32_t* inst, uint32_t k0, uint32_t k1,
                         int start, int8_t* out_s, long long* out_de, long long* out_st, const AnnealExtra& ex) {
   if (lds > 65536)
-    hipFuncSetAttribute((const void*)arpeggio_fx_kernel<P, CPL, MODE, FULL, DS>,
+    hipFuncSetAttribute((const void*)arpeggio_fx_kernel<P, CPL, MODE, FULL, DS, RNG>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((arpeggio_fx_kernel<P, CPL, MODE, FULL, DS>), grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0,
-                     k1, start, out_s, out_de, out_st, ex);
+  hipLaunchKernelGGL((arpeggio_fx_kernel<P, CPL, MODE, FULL, DS, RNG>), grid, block, lds, st, Gt, h, T, L, q, R, D,
+                     inst, k0, k1, start, out_s, out_de, out_st, ex);
 }
 
 // FULL: L is a whole number of 4 P-pass Philox blocks, so the pass loop needs no bound test
@@ -407,6 +445,15 @@ static hipError_t launch_p(const int32_t* Gt, const int32_t* h, const int32_t* T
   const int cpl = D <= ARP_WAVE ? 1 : 2;
   const size_t lds = sizeof(int32_t) * ((size_t)D * D + ARP_WAVE * cpl);
   dim3 grid(B), block(ARP_WAVE * waves);
+  if (ex.rng) {
+    // the production engine's lane-shared Chord: the parallel pass with xorshift lane streams
+    const bool full = L % (4 * P) == 0;
+    if (cpl == 1 && full) launch_pcmf<P, 1, 1, true, 0, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
+    else if (cpl == 1) launch_pcmf<P, 1, 1, false, 0, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
+    else if (full) launch_pcmf<P, 2, 1, true, 0, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
+    else launch_pcmf<P, 2, 1, false, 0, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
+    return hipGetLastError();
+  }
   if (cpl == 1) {
     if (mode == 0) launch_pcm<P, 1, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
     else launch_pcm<P, 1, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);

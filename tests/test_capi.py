@@ -98,3 +98,14 @@ def test_detect_workspace_and_argument_errors_without_gpu():
     rc = L.arp_detect_adaptive(p, p, None, 0.1, 10, 64, 64, 8, p, 8, 4, p, 8, 16, 0, 40.0, p, 0, 0, None, 1, None, p,
                                None, p, 1 << 30, None)
     assert rc == -1 and b'M must be' in L.arp_last_error()
+
+
+def test_chord_lane_argument_errors_without_gpu():
+    """arp_anneal_chord_lane refuses a bad copy count, dimension or stream choice before any HIP call."""
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    args = lambda P, D, rng: (None, None, None, None, 8, 0, P, 2, D, 1, None, 0, 0, rng, None, None, None, None)
+    assert L.arp_anneal_chord_lane(*args(3, 16, 1)) == -1 and b'P = 3' in L.arp_last_error()
+    assert L.arp_anneal_chord_lane(*args(2, 129, 1)) == -3
+    assert L.arp_anneal_chord_lane(*args(2, 16, 2)) == -1 and b'rng' in L.arp_last_error()
+    assert L.arp_anneal_chord_lane(*args(2, 16, 1)) == -1 and b'null' in L.arp_last_error()
