@@ -59,7 +59,7 @@ SIGNATURES = {
                                _P, _P],
     'arp_certify': [_P, _P, _f, _f, _i32, _P, _P, _P],
     'arp_anneal_chord_lane': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _P, _P, _P,
-                              _P],
+                              _P, _P, _P],
     'arp_mmse_sic': [_P, _P, _P, _f, _i32, _i32, _i32, _i32, _i32, _P, _P],
     'arp_mdi_radius': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _P, _P, _P, _P],
     'arp_mdi_build': [_P, _P, _P, _P, _i32, _i32, _i32, _f, _P, _P, _P],

@@ -136,6 +136,6 @@ def chord_ref_device(G, h, P, T, c, q, R, seed, inst0=0, rng='xs', inst=None, tr
     out_st = torch.empty(B, R, 2, dtype=torch.int64, device='cuda')
     _lib.check(_lib.load().arp_anneal_chord_lane(
         _lib.ptr(Gt), _lib.ptr(hh), _lib.ptr(Tt), _lib.ptr(Ct), len(Tl), int(q), int(P), int(R), int(D), int(B),
-        _lib.ptr(it), seed & M32, (seed >> 32) & M32, 1 if rng == 'xs' else 0, _lib.ptr(out_s), _lib.ptr(out_de),
-        _lib.ptr(out_st), _lib.stream_ptr()), 'chord_lane')
+        _lib.ptr(it), seed & M32, (seed >> 32) & M32, 1 if rng == 'xs' else 0, None, None, _lib.ptr(out_s),
+        _lib.ptr(out_de), _lib.ptr(out_st), _lib.stream_ptr()), 'chord_lane')
     return out_s, out_de, out_st

@@ -427,19 +427,20 @@ extern "C" int arp_mdi_build(const float* H, const float* y, const int8_t* xg, c
 
 extern "C" int arp_anneal_chord_lane(const int32_t* Gt, const int32_t* h, const int32_t* T, const int32_t* Cq, int L,
                                      int q, int P, int R, int D, int B, const uint32_t* inst, uint32_t k0,
-                                     uint32_t k1, int rng, int8_t* out_s, int64_t* out_de, int64_t* out_st,
-                                     void* stream) {
+                                     uint32_t k1, int rng, const int32_t* list, const int32_t* count, int8_t* out_s,
+                                     int64_t* out_de, int64_t* out_st, void* stream) {
   if (P < 1 || P > 6 || P % 2) return fail(ARP_E_ARG, "chord_lane: P = %d, expected 2, 4 or 6", P);
   if (D < 1 || D > 128) return fail(ARP_E_RANGE, "chord_lane: D = %d outside 1..128", D);
   if (L < 1 || R < 1 || B < 0) return fail(ARP_E_ARG, "chord_lane: L = %d, R = %d, B = %d", L, R, B);
   if (rng != 0 && rng != 1) return fail(ARP_E_ARG, "chord_lane: rng = %d, expected 0 (Philox) or 1 (xorshift)", rng);
+  if (!list != !count) return fail(ARP_E_ARG, "chord_lane: list and count go together");
   if (B == 0) return ok();
   if (!Gt || !h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "chord_lane: null pointer");
   const int waves = arpeggio_waves(R, D);
   if (!block_fits(waves, ARP_ARPEGGIO_MAX_THREADS))
     return fail(ARP_E_ARG, "chord_lane: %d anneals per workgroup (%d threads) exceed the kernel's bound of %d threads",
                 waves, ARP_WAVE * waves, ARP_ARPEGGIO_MAX_THREADS);
-  arp::AnnealExtra ex{nullptr, nullptr, nullptr, nullptr, 0, 0, Cq, rng};
+  arp::AnnealExtra ex{nullptr, nullptr, list, count, 0, 0, Cq, rng};
   return hip_check(arp::launch_arpeggio_ex(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, 0, 1, out_s, (long long*)out_de,
                                            (long long*)out_st, waves, ex, (hipStream_t)stream),
                    "chord_lane launch");

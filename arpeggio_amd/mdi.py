@@ -7,10 +7,13 @@ around the guess (arp_mdi_build), anneals them with ceil(Ns / t) first-accept an
 the group's frame list, 2K copies, per-call Philox key) and keeps a candidate only where it is strictly closer than the
 guess's best so far (arp_decode_select_list with the guess's levels and distance as the prior). After the stages, the
 better guess (first on ties) is the decision. All arrays stay on the device; the host reads the radii once per stage.
+With the production Chord engine as the solver (pick prod_fx_xs_<Ns>), the same sub-problems go to
+arp_anneal_chord_lane (Sk 2K steps, xorshift32 lane streams, key without the tag), as the reference's x3_prod runs.
 
 Reference: arpeggio/mimo/mimo.py:546-602 (mdi), arpeggio/sim/local_code_a513/run_a513.py:43-130 (mdi_logged and its
 fixed-point Arpeggio solver: key (seed_base + ci KEY_STEP + KEY_TAG) mod 2^64 for solver call ci, instance word = frame
-index in the batch, schedule S sweeps x 2K passes), arpeggio/mimo/mdi_solvers.py:16-20 (Rmax, S, guesses)."""
+index in the batch, schedule S sweeps x 2K passes; its 'prod' solver, run_a513.py:97-122, and x3_prod.py:90-112),
+arpeggio/mimo/mdi_solvers.py:16-20 (Rmax, S, guesses, MDICHORD)."""
 import math
 
 import numpy as np
@@ -18,42 +21,60 @@ import torch
 
 from . import _lib
 from .arpeggiofx import schedule_check
-from .chordfx import QSCALE
+from .chordfx import QSCALE, to_fixed
 from .detector import MASK, _f32, _n0_arg, arpeggio_schedule
 
 M64 = 0xFFFFFFFFFFFFFFFF
+_DUMMY = dict(J=torch.zeros(1, 2, 2), h=torch.zeros(1, 2), h0=torch.zeros(1, 2))
 KEY_MUL = 0x9E3779B97F4A7C15     # seed base = draw seed x this (run_a513.py:99)
 KEY_STEP = 0xD1B54A32D192ED03    # per solver call (run_a513.py:28)
-KEY_TAG = 0xA513                 # first-accept stream tag (run_a513.py:29)
+KEY_TAG = 0xA513                 # first-accept stream tag (run_a513.py:29); the Chord engine's keys have none
 # Singh et al. 2024 Table I as the reference configures it (mdi_solvers.py:16-20): (users, M) -> Rmax, stages, guesses
 MDI_CFG = {(16, 16): dict(Rmax=3, S=8, guesses=('mmse', 'sic')), (32, 16): dict(Rmax=3, S=8, guesses=('mmse', 'sic')),
            (64, 16): dict(Rmax=3, S=8, guesses=('mmse', 'sic')), (32, 4): dict(Rmax=1, S=1, guesses=('mmse',))}
+# the production Chord engine's schedule in MDI (mdi_solvers.py:18): Sk steps per copy, T from t0 to t1, pinning q
+MDICHORD = dict(Sk=128, t0=0.3, t1=0.3, q=0.4, kappa=0.0)
 
 
 def parse_pick(pick):
-    """abl_far_<S>_<T0>_<T1>_<q>_<start>_<Ns> (the reference's method names) -> dict (run_a513.py:34-38)."""
+    """abl_far_<S>_<T0>_<T1>_<q>_<start>_<Ns> (the reference's method names) -> dict (run_a513.py:34-38);
+    prod_fx_xs_<Ns> (the production Chord engine, x3_prod.py:129) -> dict(solver='prod', Ns)."""
+    if pick.startswith('prod_fx_xs_'):
+        return dict(solver='prod', Ns=int(pick[len('prod_fx_xs_'):]))
     a = pick.replace('arpeggiofx_', 'abl_far_').split('_')
     if len(a) != 8 or a[0] != 'abl' or a[1] != 'far':
         raise ValueError('pick must look like abl_far_<S>_<T0>_<T1>_<q>_<start>_<Ns>: %r' % pick)
-    return dict(S=int(a[2]), T0=float(a[3]), T1=float(a[4]), q=float(a[5]), start=a[6], Ns=int(a[7]))
+    return dict(solver='arpeggio', S=int(a[2]), T0=float(a[3]), T1=float(a[4]), q=float(a[5]), start=a[6],
+                Ns=int(a[7]))
 
 
 class MDIDetector:
-    """MDI with the fixed-point first-accept solver for batches of B frames of nt users (M-QAM).
+    """MDI for batches of B frames of nt users (M-QAM) with the fixed-point first-accept solver or the production
+    Chord engine (lane-shared Chord with xorshift32 lane streams).
 
-    pick: the solver's schedule (abl_far_* string or dict S, T0, T1, q, start, Ns). cfg: Rmax, S (stages), guesses
-    (default: MDI_CFG). Counters per K ([flips, tests, anneals]) accumulate in self.cnt over detect() calls."""
+    pick: abl_far_* / arpeggiofx_* string or dict (S, T0, T1, q, start, Ns) for first-accept; prod_fx_xs_<Ns> or
+    dict(solver='prod', Ns) for the production Chord (schedule MDICHORD, x3_prod.py:80-84). cfg: Rmax, S (stages),
+    guesses (default: MDI_CFG). Counters per K accumulate in self.cnt over detect() calls: [flips, tests, anneals]
+    for first-accept, [steps, flips, anneals] for Chord (the sum of the first two is the modeled cycles, see
+    arpeggio_amd.cycles); self.ann_k (B, Rmax) holds the last batch's anneals per frame and K."""
 
     def __init__(self, B, nt, M, pick, cfg=None, Q=QSCALE):
         self.B, self.nt, self.M, self.D, self.Mr = B, nt, M, 2 * nt, 2 * nt
         self.pk = parse_pick(pick) if isinstance(pick, str) else dict(pick)
+        self.pk.setdefault('solver', 'arpeggio')
         self.cfg = dict(cfg or MDI_CFG[(nt, M)])
         self.Q = float(Q)
         self.lib = _lib.load()
         dev, D = 'cuda', self.D
-        self.T = {}
+        self.T, self.C = {}, {}
         for K in range(1, self.cfg['Rmax'] + 1):
-            T, qi = arpeggio_schedule(self.pk['S'], self.pk['T0'], self.pk['T1'], self.pk['q'], K, Q)
+            if self.pk['solver'] == 'prod':
+                mc = MDICHORD
+                F = to_fixed(_DUMMY, mc['Sk'] * 2 * K, mc['t0'], mc['t1'], mc['q'], mc['kappa'], Q=Q)
+                T, qi = F['T'].cpu().tolist(), F['q']
+                self.C[K] = F['c'].to(device=dev, dtype=torch.int32).contiguous()
+            else:
+                T, qi = arpeggio_schedule(self.pk['S'], self.pk['T0'], self.pk['T1'], self.pk['q'], K, Q)
             schedule_check(T, qi, 2 * K, D, Q)
             self.T[K] = torch.tensor(T, dtype=torch.int32, device=dev)
             self.q = qi
@@ -103,10 +124,15 @@ class MDIDetector:
         de = torch.empty(n, na, dtype=torch.int64, device='cuda')
         st = torch.empty(n, na, 2, dtype=torch.int64, device='cuda')
         T = self.T[K]
-        _lib.check(L.arp_anneal_arpeggio_list(P_(self.Gi), P_(self.hi), P_(T), T.numel(), self.q, 2 * K, na, D, n,
-                                              P_(self.inst), key & MASK, (key >> 32) & MASK,
-                                              0 if self.pk['start'] == 'rand' else 1, None, 0, 0, P_(idx),
-                                              P_(self.count), P_(spins), P_(de), P_(st), None, s), 'anneal (MDI)')
+        if self.pk['solver'] == 'prod':
+            _lib.check(L.arp_anneal_chord_lane(P_(self.Gi), P_(self.hi), P_(T), P_(self.C[K]), T.numel(), self.q, 2 * K,
+                                               na, D, n, P_(self.inst), key & MASK, (key >> 32) & MASK, 1, P_(idx),
+                                               P_(self.count), P_(spins), P_(de), P_(st), s), 'Chord (MDI)')
+        else:
+            _lib.check(L.arp_anneal_arpeggio_list(P_(self.Gi), P_(self.hi), P_(T), T.numel(), self.q, 2 * K, na, D, n,
+                                                  P_(self.inst), key & MASK, (key >> 32) & MASK,
+                                                  0 if self.pk['start'] == 'rand' else 1, None, 0, 0, P_(idx),
+                                                  P_(self.count), P_(spins), P_(de), P_(st), None, s), 'anneal (MDI)')
         return spins, st
 
     def detect(self, H, y, N0, draw_seed, ltrue=None, counters=True):
@@ -118,8 +144,10 @@ class MDIDetector:
         cfg = self.cfg
         seed_base = (int(draw_seed) * KEY_MUL) & M64
         ci = 0
+        prod = self.pk['solver'] == 'prod'
         cur = self.guesses(H, y, N0)
         ann = np.zeros(B)
+        self.ann_k = np.zeros((B, cfg['Rmax']))
         for st in range(cfg['S']):
             Rd = {}
             for g in cfg['guesses']:
@@ -139,7 +167,7 @@ class MDIDetector:
                     for t in np.unique(tot[need]).tolist():
                         idx_h = np.flatnonzero(need & (tot == t))
                         na = math.ceil(self.pk['Ns'] / t)
-                        key = (seed_base + ci * KEY_STEP + KEY_TAG) & M64
+                        key = (seed_base + ci * KEY_STEP + (0 if prod else KEY_TAG)) & M64
                         ci += 1
                         idx = torch.as_tensor(idx_h.astype(np.int32)).cuda()
                         spins, stt = self.solve(H, y, cur[g], idx, K, na, key)
@@ -149,11 +177,16 @@ class MDIDetector:
                             self.Mr, M, 1, None, _lib.ptr(idx), _lib.ptr(self.count), _lib.ptr(bl), _lib.ptr(bd),
                             _lib.ptr(bl), None, _lib.ptr(bd), _lib.stream_ptr()), 'decode (MDI)')
                         ann[idx_h] += na
+                        self.ann_k[idx_h, K - 1] += na
                         if counters:
                             v = self.cnt.setdefault(K, [0, 0, 0])
                             sts = stt.sum((0, 1)).cpu()
-                            v[0] += int(sts[0])
-                            v[1] += int(sts[1])
+                            if prod:   # [steps, flips, anneals]: L steps per anneal
+                                v[0] += self.T[K].numel() * na * len(idx_h)
+                                v[1] += int(sts[0])
+                            else:      # [flips, tests, anneals]
+                                v[0] += int(sts[0])
+                                v[1] += int(sts[1])
                             v[2] += na * len(idx_h)
                 nxt[g] = bl
             cur = nxt

@@ -62,11 +62,15 @@ int arp_anneal_arpeggio(const int32_t *Gt, const int32_t *h, const int32_t *T, i
  * (Gt (B, D, D) int32 transposed Gram = Gram, h (B, D)), P copies, L steps with T and Cq (Onsager c_k, may be NULL)
  * per step, pinning q; step k tests copy k mod P on one snapshot and commits every accepted lane. rng 1: the
  * xorshift32 lane streams of the production engine (seeded from k0, k1, inst, r by fmix32; RB = 16); rng 0: Philox
- * (then with Cq = NULL it equals arp_anneal_arpeggio's mode 1). Outputs as arp_anneal_arpeggio (out_st = flips,
- * steps). Used by the x3_prod-style MDI runs (engine_py.solve_batch) in the reference. */
+ * (then with Cq = NULL it equals arp_anneal_arpeggio's mode 1). list/count (both NULL, or both set): slot i < *count
+ * anneals frame list[i] (problem and instance word at the frame's index, outputs at slot i), as
+ * arp_anneal_arpeggio_list. Outputs as arp_anneal_arpeggio (out_st = flips, comparator passes). Replaces
+ * engine_py.solve_batch (arpeggio/sim/x3model/engine_py.py:116-133) as the production-Chord solver of the MDI runs
+ * (arpeggio/sim/x3_prod.py:97-112, run_a513.py:119-122); the engine's modeled cost per anneal is L + flips. */
 int arp_anneal_chord_lane(const int32_t *Gt, const int32_t *h, const int32_t *T, const int32_t *Cq, int L, int q, int P,
-                          int R, int D, int B, const uint32_t *inst, uint32_t k0, uint32_t k1, int rng, int8_t *out_s,
-                          int64_t *out_de, int64_t *out_st, void *stream);
+                          int R, int D, int B, const uint32_t *inst, uint32_t k0, uint32_t k1, int rng,
+                          const int32_t *list, const int32_t *count, int8_t *out_s, int64_t *out_de, int64_t *out_st,
+                          void *stream);
 
 /* Fixed-point Chord annealer. Replaces `chord_m{1,2,3,4,6,8}` (arpeggio/mimo/chord_kernel.cu:177-184, same argument
  * order and meaning; launched by Kernel.chord, arpeggio/mimo/chordfx.py:189-208), bit-exact to chordfx.chord_fx.

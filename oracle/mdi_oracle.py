@@ -8,6 +8,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/ and the benchmark's CPU-baseline le
                            chord_ref.compress (arpeggio/sim/x3model/chord_ref.py:47-59), in the compressed form
   mdi                      arpeggio/mimo/mimo.py:546-602 and arpeggio/sim/local_code_a513/run_a513.py:43-80
   ArpeggioMDISolver        arpeggio/sim/local_code_a513/run_a513.py:90-130 (per-call Philox keys, fixed point)
+  ChordMDISolver           arpeggio/sim/x3_prod.py:80-112 (the production Chord engine in MDI, via chord_lane)
   mmse_soft_llr, list_llr  arpeggio/mimo/mimo.py:111-129, 434-462
   fmix32, xs32, lane_seeds arpeggio/sim/x3model/chord_ref.py:17-45
 """
@@ -205,6 +206,38 @@ class ArpeggioMDISolver:
         v = self.cnt.setdefault(K, [0, 0, 0])
         v[0] += int(st[..., 0].sum())
         v[1] += int(st[..., 1].sum())
+        v[2] += int(na * len(idx))
+        return s.astype(np.float64)
+
+
+class ChordMDISolver:
+    """The production Chord engine as the MDI solver (x3_prod.py:80-112; run_a513.py:97-122 'prod'): call ci anneals
+    its frames with the lane-shared Chord (chord_lane, xorshift32 lane streams) under key (seed_base + ci KEY_STEP)
+    mod 2^64 (no tag), instance word = frame index, 2K copies, the MDICHORD schedule of Sk 2K steps (T, c, q from
+    to_fixed on an empty problem, as the reference tabulates them). Keeps per-K counters [steps, flips, anneals], the
+    order the reference logs the engine's D-ops and F-ops."""
+
+    def __init__(self, draw_seed, nthreads=None, cfg=MDICHORD):
+        self.base = mdi_seed_base(draw_seed)
+        self.cfg = cfg
+        self.calls, self.nthreads = 0, nthreads
+        self.cnt = {}
+
+    def key(self, ci):
+        return (self.base + ci * KEY_STEP) & M64
+
+    def __call__(self, H, y, xg, K, na, idx, stage, guess):
+        ci = self.calls
+        self.calls += 1
+        G, h, _ = ddi_fixed(H, y, xg, K)
+        c = self.cfg
+        F = orc.to_fixed(np.zeros((1, 2, 2)), np.zeros((1, 2)), np.zeros((1, 2)), c['Sk'] * 2 * K, c['t0'], c['t1'],
+                         c['q'], c['kappa'])
+        s, de, fl = orc.chord_lane(G, h, 2 * K, na, F['T'], F['c'], F['q'], idx, self.key(ci), 'xs',
+                                   nthreads=self.nthreads)
+        v = self.cnt.setdefault(K, [0, 0, 0])
+        v[0] += int(len(F['T']) * na * len(idx))
+        v[1] += int(fl.sum())
         v[2] += int(na * len(idx))
         return s.astype(np.float64)
 

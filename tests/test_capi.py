@@ -104,7 +104,8 @@ def test_chord_lane_argument_errors_without_gpu():
     """arp_anneal_chord_lane refuses a bad copy count, dimension or stream choice before any HIP call."""
     from arpeggio_amd import _lib
     L = _lib.load()
-    args = lambda P, D, rng: (None, None, None, None, 8, 0, P, 2, D, 1, None, 0, 0, rng, None, None, None, None)
+    args = lambda P, D, rng: (None, None, None, None, 8, 0, P, 2, D, 1, None, 0, 0, rng, None, None, None, None, None,
+                             None)
     assert L.arp_anneal_chord_lane(*args(3, 16, 1)) == -1 and b'P = 3' in L.arp_last_error()
     assert L.arp_anneal_chord_lane(*args(2, 129, 1)) == -3
     assert L.arp_anneal_chord_lane(*args(2, 16, 2)) == -1 and b'rng' in L.arp_last_error()

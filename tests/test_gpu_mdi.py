@@ -152,3 +152,62 @@ def test_mdi_vs_reference_fixtures(mdo):
     assert (ours == theirs).mean() >= 0.99
     for K in (1, 2, 3):
         assert abs(det.cnt[K][2] - cnt_ref[K - 1, 2]) <= 0.03 * cnt_ref[K - 1, 2]
+
+
+def test_mdi_chord_chain_vs_oracle(orc, mdo):
+    """MDI with the production Chord engine (lane-shared Chord, xorshift32 lane streams) on the GPU against the oracle's
+    chain (numpy float64 + the C lane-shared Chord, bit-equal to the reference's engine model) on one 32x32 batch of the
+    reference's run: per-use decisions agree on >= 99 %; counters [steps, flips, anneals] within 1 % at K = 1 and the
+    cost per anneal within 5 % at every K (a radius floor decided by rounding moves a frame between groups, and so
+    shifts the few K = 2, 3 sub-problems of one batch and their per-call keys)."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.mdi import MDIDetector
+    B = 300
+    b = frames.draw(102, 22.0, 0, 1000, 32, 16, dev='cuda')
+    b = {k: (v[:B] if torch.is_tensor(v) else v) for k, v in b.items()}
+    ds = frames.draw_seed(102, 22.0, 0)
+    det = MDIDetector(B, 32, 16, 'prod_fx_xs_8')
+    lh, ann, be = det.detect(b['H'], b['y'], b['N0'], ds, ltrue=b['l'])
+    torch.cuda.synchronize()
+    sol = mdo.ChordMDISolver(ds)
+    lo, anno = mdo.mdi(b['H'].cpu().numpy(), b['y'].cpu().numpy(), b['N0'], 16, sol, 8, 3, 8, ('mmse', 'sic'))
+    agree = (lh.cpu().numpy() == lo).all(1)
+    say('per-use agreement', agree.mean(), 'anneals equal', (ann == anno).mean(), 'counters', det.cnt, sol.cnt)
+    assert agree.mean() >= 0.99 and (ann == anno).mean() >= 0.9
+    for i in range(3):
+        assert abs(det.cnt[1][i] - sol.cnt[1][i]) <= 0.01 * sol.cnt[1][i]
+    from arpeggio_amd import cycles
+    kd, ko = cycles.cost_per_anneal(det.cnt), cycles.cost_per_anneal(sol.cnt)
+    for K in ko:
+        assert kd[K] == pytest.approx(ko[K], rel=0.05)
+
+
+def test_mdi_modeled_budget_vs_reference():
+    """First-accept Arpeggio against the production Chord inside MDI at 32 users, 22 dB (the reference's point P2,
+    data/results/a513/analysis.json) on its first 30 batches: each solver's modeled cost per anneal per K within 3 % of
+    the reference's (tests + flips; L + flips), x_F within 3 %, and the reference's paired BER ratio (0.98) inside
+    our paired bootstrap interval."""
+    from arpeggio_amd import cycles, frames
+    from arpeggio_amd.mdi import MDIDetector
+    from arpeggio_amd.stats import boot_ratio
+    g = json.load(open(os.path.join(GOLDEN, 'a513_cycles.json')))['P2']
+    rec = json.load(open(os.path.join(GOLDEN, 'a513_fixed.json')))['x32']
+    nb = 30
+    fx = MDIDetector(rec['m'], 32, 16, rec['pick'])
+    pr = MDIDetector(rec['m'], 32, 16, 'prod_fx_xs_8')
+    ef, ep = [], []
+    for bi in range(nb):
+        b = frames.draw(rec['seed'], rec['snr'], bi, rec['m'], 32, 16, dev='cuda')
+        ds = frames.draw_seed(rec['seed'], rec['snr'], bi)
+        ef.append(fx.detect(b['H'], b['y'], b['N0'], ds, ltrue=b['l'])[2].cpu().numpy())
+        ep.append(pr.detect(b['H'], b['y'], b['N0'], ds, ltrue=b['l'])[2].cpu().numpy())
+    ef, ep = np.concatenate(ef), np.concatenate(ep)
+    r, lo, hi = boot_ratio(ef, ep)
+    x, xk, kf, kp = cycles.x_f(fx.cnt, pr.cnt)
+    say('BER fx %.3e prod %.3e ratio %.3f [%.3f, %.3f]; x_F %.3f %s; cost fx %s prod %s' % (
+        ef.sum() / ef.size / 128, ep.sum() / ep.size / 128, r, lo, hi, x, xk, kf, kp))
+    for K in ('1', '2', '3'):
+        assert kf[K] == pytest.approx(g['cost_per_anneal']['fx'][K], rel=0.03)
+        assert kp[K] == pytest.approx(g['cost_per_anneal']['prod'][K], rel=0.03)
+    assert x == pytest.approx(g['x_F'], rel=0.03) and cycles.within_budget(x)
+    assert lo <= g['fx_prod']['ratio'] <= hi
