@@ -133,7 +133,8 @@ __device__ __forceinline__ uint32_t xs32(uint32_t x) {
 // RNG 1: the production engine's xorshift32 lane streams (one state per coordinate, seeded from the key, instance
 // and anneal by fmix32; start spin of copy p = bit 31 - p of the first state, each pass draws u = x >> 16 from the
 // next, RB = 16 fraction bits), as arpeggio/sim/x3model/chord_ref.py:17-45 and 99-107 define them. Cq (per pass, may be
-// NULL): an Onsager term subtracted from the decision bound, B_k = q - c_k + 2 T_k (chord_ref.py:102).
+// NULL): an Onsager term subtracted from the decision bound, B_k = q - c_k + 2 T_k (chord_ref.py:102), read for RNG 1
+// and RNG 2 (Philox streams with the term); RNG 0, the plain annealers, has none.
 template <int P, int CPL, int MODE, bool FULL, int DS = 0, int RNG = 0>
 __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb,
                                                 const int32_t* __restrict__ T, int L, int q, int D, uint32_t r,
@@ -229,14 +230,14 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
 #pragma unroll
     for (int i = 0; i < 4 * P; ++i) Tb[i] = (FULL || kb + i < L) ? T[kb + i] : 0;  // FULL: L % 4P == 0
 #pragma unroll
-    for (int i = 0; i < 4 * P; ++i) Cb[i] = (Cq && (FULL || kb + i < L)) ? Cq[kb + i] : 0;
+    for (int i = 0; i < 4 * P; ++i) Cb[i] = (RNG != 0 && Cq && (FULL || kb + i < L)) ? Cq[kb + i] : 0;
 #pragma unroll
     for (int nm = 0; nm < 4; ++nm) {
 #pragma unroll
       for (int p = 0; p < P; ++p) {
         const int k = kb + nm * P + p;
         if (!FULL && k >= L) break;
-        if (RNG == 0 && nm == 0) {
+        if (RNG != 1 && nm == 0) {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             uint32_t c0 = (uint32_t)n4, c1 = (uint32_t)(p * D + c * ARP_WAVE + lane), c2 = r, c3 = inst;
@@ -252,7 +253,7 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
         // Decision x A > (q_j + B_k) 2^32 (B_k = q + 2 T_k, q_j = s w_j + 2 G_jj) as q_j <= tq with
         // tq = floor((x A - 1) / 2^32) - B_k: the high word of one 32x32+64 multiply-add with addend -1 (x A = 0
         // gives -1), then one subtraction; the documented ranges keep tq in int32.
-        const int32_t Bk = q + 2 * Tk - Cb[nm * P + p];
+        const int32_t Bk = q + 2 * Tk - (RNG != 0 ? Cb[nm * P + p] : 0);
         int32_t tq[CPL];
         if (RNG == 1) {
           // u A > (q_j + B_k) 2^16 with the 16-bit draw u of the lane's next xorshift state
@@ -445,13 +446,25 @@ static hipError_t launch_p(const int32_t* Gt, const int32_t* h, const int32_t* T
   const int cpl = D <= ARP_WAVE ? 1 : 2;
   const size_t lds = sizeof(int32_t) * ((size_t)D * D + ARP_WAVE * cpl);
   dim3 grid(B), block(ARP_WAVE * waves);
-  if (ex.rng) {
-    // the production engine's lane-shared Chord: the parallel pass with xorshift lane streams
+  if (ex.rng || ex.C) {
+    // the production engine's lane-shared Chord: the parallel pass with the Onsager term, xorshift lane streams
+    // (RNG 1) or Philox (RNG 2)
     const bool full = L % (4 * P) == 0;
-    if (cpl == 1 && full) launch_pcmf<P, 1, 1, true, 0, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
-    else if (cpl == 1) launch_pcmf<P, 1, 1, false, 0, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
-    else if (full) launch_pcmf<P, 2, 1, true, 0, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
-    else launch_pcmf<P, 2, 1, false, 0, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
+#define ARP_LANE(CPLV, F, RV) \
+  launch_pcmf<P, CPLV, 1, F, 0, RV>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, \
+                                    out_st, ex)
+    if (ex.rng) {
+      if (cpl == 1 && full) ARP_LANE(1, true, 1);
+      else if (cpl == 1) ARP_LANE(1, false, 1);
+      else if (full) ARP_LANE(2, true, 1);
+      else ARP_LANE(2, false, 1);
+    } else {
+      if (cpl == 1 && full) ARP_LANE(1, true, 2);
+      else if (cpl == 1) ARP_LANE(1, false, 2);
+      else if (full) ARP_LANE(2, true, 2);
+      else ARP_LANE(2, false, 2);
+    }
+#undef ARP_LANE
     return hipGetLastError();
   }
   if (cpl == 1) {

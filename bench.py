@@ -323,21 +323,23 @@ def mdi(args):
     """The MDI wrapper with the fixed-point first-accept solver on the reference's own batches (its draws; seeds, SNRs
     and picks of data/results/a513_fixed, copied into tests/golden/a513_fixed.npz): per-use bit errors paired with the
     reference's (bootstrap CI of the BER ratio), anneals per K against its counters, and detections/s (GPU time of the
-    detect calls, host grouping included; the draws are not timed). Then the throughput of one large batch."""
+    detect calls, host grouping included; the draws are not timed). On the same batches, the production Chord engine
+    (lane-shared Chord, xorshift32 streams, prod_fx_xs_<Ns>) in the same wrapper: the paired BER ratio of first-accept
+    to Chord and x_F, first-accept's modeled cycles per anneal (tests + flips) over Chord's (L + flips), beside the
+    reference's (data/results/a513/analysis.json, in tests/golden/a513_cycles.json). Then the throughput of one large
+    batch."""
     import torch
-    from arpeggio_amd import frames
-    from arpeggio_amd.mdi import MDIDetector
+    from arpeggio_amd import cycles, frames
+    from arpeggio_amd.mdi import MDIDetector, parse_pick
     from arpeggio_amd.stats import boot_ratio
     gd = os.path.join(ROOT, 'tests', 'golden')
     z = np.load(os.path.join(gd, 'a513_fixed.npz'), allow_pickle=False)
     idx = json.load(open(os.path.join(gd, 'a513_fixed.json')))
-    out = {}
-    for tag, nb in (('x32', args.mdi_batches), ('x16', max(1, args.mdi_batches // 5)),
-                    ('x64', max(1, args.mdi_batches // 2))):
-        rec = idx[tag]
-        nb = min(nb, rec['batches'])
-        det = MDIDetector(rec['m'], rec['nt'], 16, rec['pick'])
-        ours, wall = [], 0.0
+    gcy = json.load(open(os.path.join(gd, 'a513_cycles.json')))
+    point = {'x16': 'P1', 'x32': 'P2', 'x64': 'P4'}
+
+    def run(det, rec, nb):
+        errs, wall = [], 0.0
         for bi in range(nb):
             b = frames.draw(rec['seed'], rec['snr'], bi, rec['m'], rec['nt'], 16, dev='cuda')
             torch.cuda.synchronize()
@@ -345,8 +347,16 @@ def mdi(args):
             _, _, be = det.detect(b['H'], b['y'], b['N0'], frames.draw_seed(rec['seed'], rec['snr'], bi), ltrue=b['l'])
             torch.cuda.synchronize()
             wall += time.perf_counter() - t0
-            ours.append(be.cpu().numpy().astype(np.int64))
-        ours = np.concatenate(ours)
+            errs.append(be.cpu().numpy().astype(np.int64))
+        return np.concatenate(errs), wall
+
+    out = {}
+    for tag, nb in (('x32', args.mdi_batches), ('x16', max(1, args.mdi_batches // 5)),
+                    ('x64', max(1, args.mdi_batches // 2))):
+        rec = idx[tag]
+        nb = min(nb, rec['batches'])
+        det = MDIDetector(rec['m'], rec['nt'], 16, rec['pick'])
+        ours, wall = run(det, rec, nb)
         theirs = z['%s/biterr' % tag][:nb].reshape(-1).astype(np.int64)
         r, lo, hi = boot_ratio(ours, theirs)
         nbits = ours.size * 4 * rec['nt']
@@ -361,6 +371,25 @@ def mdi(args):
                         batch=rec['m'])
         log('MDI %s: BER %.3e vs reference %.3e, ratio %.3f [%.3f, %.3f], %.0f detections/s' % (
             tag, out[tag]['ber'], out[tag]['reference_ber'], r, lo, hi, out[tag]['value']))
+        # the production Chord engine on the same batches: the comparison within Chord's modeled budget
+        pm = 'prod_fx_xs_%d' % parse_pick(rec['pick'])['Ns']
+        dc = MDIDetector(rec['m'], rec['nt'], 16, pm)
+        ch, wall_c = run(dc, rec, nb)
+        rc, lc, hc = boot_ratio(ours, ch)
+        x, xk, kf, kp = cycles.x_f(det.cnt, dc.cnt)
+        g = gcy[point[tag]]
+        out[tag]['chord'] = dict(
+            method=pm, ber=float(ch.sum()) / nbits, value=ch.size / wall_c, unit='detections/s',
+            ratio_first_accept_to_chord=rc, ci=[lc, hc], x_F=x, x_F_per_K=xk, cycles_per_anneal_first_accept=kf,
+            cycles_per_anneal_chord=kp, within_budget=cycles.within_budget(x),
+            reference=dict(ratio=g['fx_prod']['ratio'], ci=[g['fx_prod']['lo'], g['fx_prod']['hi']], x_F=g['x_F'],
+                           cycles_per_anneal_first_accept=g['cost_per_anneal']['fx'],
+                           cycles_per_anneal_chord=g['cost_per_anneal']['prod'], uses=g['fx_prod']['uses'],
+                           chord_ber=g['fx_prod']['b_ber']))
+        log('MDI %s within Chord\'s modeled budget: first-accept / Chord BER %.3f [%.3f, %.3f] (reference %.3f '
+            '[%.3f, %.3f]), x_F %.3f (reference %.3f), Chord %.0f detections/s' % (
+                tag, rc, lc, hc, g['fx_prod']['ratio'], g['fx_prod']['lo'], g['fx_prod']['hi'], x, g['x_F'],
+                ch.size / wall_c))
     # one large batch (throughput; the per-batch keys of a 65,536-frame batch have no reference counterpart)
     rec = idx['x32']
     Bl = args.mdi_large
