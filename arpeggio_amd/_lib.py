@@ -31,6 +31,7 @@ class BatchClass(ctypes.Structure):
 SIGNATURES = {
     'arp_philox_kat': [_P, _P, _i32, _P],
     'arp_set_anneal_waves': [_i32],
+    'arp_set_front_end_form': [_i32],
     'arp_anneal_arpeggio': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _i32, _P, _P, _P,
                             _P],
     'arp_anneal_chord': [_P, _P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _i32, _P, _P, _P,

@@ -81,6 +81,12 @@ extern "C" int arp_set_anneal_waves(int waves) {
   return ok();
 }
 
+extern "C" int arp_set_front_end_form(int form) {
+  if (form != 0 && form != 1) return fail(ARP_E_ARG, "set_front_end_form: %d, expected 0 or 1", form);
+  arp::set_front_end_form(form);
+  return ok();
+}
+
 extern "C" int arp_philox_kat(const uint32_t* in, uint32_t* out, int n, void* stream) {
   if (n < 0) return fail(ARP_E_ARG, "philox_kat: n = %d < 0", n);
   if (n == 0) return ok();

@@ -503,8 +503,8 @@ static void launch_w64(const float* H, const float* y, const float* N0v, float N
   hipLaunchKernelGGL((front_end_w64_kernel<DP>), dim3(B), dim3(64), lds, st, H, y, N0v, N0s, Mr, D, M, K, Q, vec4, o);
 }
 
-hipError_t launch_front_end(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M,
-                            int K, float Q, const FrontOut& o, hipStream_t st) {
+hipError_t launch_front_end_sweep(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D,
+                                  int M, int K, float Q, const FrontOut& o, hipStream_t st) {
   if (D <= 64 && Mr <= 64) {
     const int DP = (D + 15) & ~15;
     if (DP == 16) launch_w64<16>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);

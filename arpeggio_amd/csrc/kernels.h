@@ -55,8 +55,13 @@ hipError_t launch_chord_fx(const int16_t* J, const int32_t* h, const int32_t* h0
                            const int32_t* Cq, int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1,
                            int inst0, int mode, int32_t* out_e, int8_t* out_s, unsigned long long* stats, int waves,
                            size_t max_lds, hipStream_t st);
+// Front end: the matrix-core block sweep (front_end_mfma.hip) for D, Mr <= 64 without the expanded Chord inputs, the
+// register sweep (front_end.hip) otherwise; set_front_end_form(1) forces the register sweep everywhere.
 hipError_t launch_front_end(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M,
                             int K, float Q, const FrontOut& o, hipStream_t st);
+hipError_t launch_front_end_sweep(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D,
+                                  int M, int K, float Q, const FrontOut& o, hipStream_t st);
+void set_front_end_form(int form);
 hipError_t launch_decode_select(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R,
                                 int P, int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat,
                                 int32_t* biterr, float* dist_out, hipStream_t st);

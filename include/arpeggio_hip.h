@@ -37,6 +37,11 @@ int arp_abi_version(void);
  * A tuning knob with no reference counterpart (the reference launches 128-thread blocks, arpeggiofx.py:39-42). */
 int arp_set_anneal_waves(int waves);
 
+/* Front-end schedule for D, Mr <= 64 (no counterpart in the reference, which has one float front end): 0 (default)
+ * the matrix-core block sweep, 1 the register sweep. Both compute the same quantities; the results agree to float32
+ * rounding. Process-wide, like arp_set_anneal_waves. */
+int arp_set_front_end_form(int form);
+
 /* Philox4x32-10 known-answer kernel: in (n, 6) uint32 rows (counter[4], key[2]) -> out (n, 4).
  * Replaces the reference's device KAT `philox_kat` (arpeggio/mimo/chord_kernel.cu:26-32), driven by
  * Kernel.philox_kat (arpeggio/mimo/chordfx.py:180-187). */
