@@ -37,7 +37,7 @@ SIGNATURES = {
     'arp_anneal_chord': [_P, _P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _i32, _P, _P, _P,
                          _P],
     'arp_front_end': [_P, _P, _P, ctypes.c_float, _i32, _i32, _i32, _i32, _i32, ctypes.c_float, _P, _P, _P, _P, _P, _P,
-                      _P, _P, _P, _P, _P, _P, _P, _i32, _P],
+                      _P, _P, _P, _P, _P, _P, _P, _i32, _P, _P],
     'arp_anneal_arpeggio_ws': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _P, _i32, _P, _P, _P, _P,
                                _P],
     'arp_anneal_arpeggio_batched': [_P, _i32, _i32, _u32, _u32, _P, _i32, _i32, _P],
@@ -54,8 +54,8 @@ SIGNATURES = {
     'arp_anneal_chord_f32': [_P, _P, _P, _P, _P, ctypes.c_float, _i32, _i32, _i32, _i32, _i32, _u32, _u32, _i32, _P, _P,
                              _P, _P, _P],
     'arp_decode_select': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _P, _P, _P, _P, _P],
-    'arp_anneal_arpeggio_list': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _P, _i32, _i32,
-                                 _P, _P, _P, _P, _P, _P, _P],
+    'arp_anneal_arpeggio_list': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _P, _i32,
+                                 _i32, _P, _P, _P, _P, _P, _P, _P],
     'arp_decode_select_list': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _P, _P, _P, _P, _P, _P, _P,
                                _P, _P],
     'arp_certify': [_P, _P, _f, _f, _i32, _P, _P, _P],

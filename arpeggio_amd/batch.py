@@ -129,7 +129,7 @@ class BatchedDetector:
             n0t, n0s = _n0_arg(f['N0'], c['B'])
             _lib.check(L.arp_front_end(P_(f['H']), P_(f['y']), P_(n0t), n0s, c['B'], c['Mr'], c['D'], c['M'], 1,
                                        self.Q, P_(c['xg']), P_(c['Gi']), P_(c['hi']), P_(c['scale']), None, None,
-                                       None, None, None, None, None, None, None, 0, s), 'front_end')
+                                       None, None, None, None, None, None, None, 0, None, s), 'front_end')
         self.anneal(stream)
         for c, f in zip(self.cls, frames):
             lt = f.get('lt') if with_errors else None

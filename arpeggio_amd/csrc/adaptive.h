@@ -18,6 +18,7 @@ struct AnnealExtra {
   int r0;
   const int32_t* C;
   int rng;
+  const int16_t* Gh;  // the Gram as int16 in place of Gt (the front end's Gh output)
 };
 
 // Optional arguments of the decode kernels: a frame list (as AnnealExtra; spins are read from slot i, everything else

@@ -392,8 +392,22 @@ __global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) void arpeggio_fx_kernel(c
     b = ex.list[slot];
   }
   const int nG = D * D;
-  const int32_t* g = Gt + (size_t)b * nG;
-  for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? 4 * g[x] : 0;
+  if (ex.Gh) {
+    const int16_t* g = ex.Gh + (size_t)b * nG;
+    if ((nG & 3) == 0) {  // four entries per 8-byte load (frames start on 8-byte boundaries)
+      for (int x4 = threadIdx.x; x4 < nG / 4; x4 += blockDim.x) {
+        const uint2 w = reinterpret_cast<const uint2*>(g)[x4];
+        reinterpret_cast<int4*>(sG)[x4] = make_int4(4 * (int32_t)(int16_t)(w.x & 0xFFFFu), 4 * ((int32_t)w.x >> 16),
+                                                    4 * (int32_t)(int16_t)(w.y & 0xFFFFu), 4 * ((int32_t)w.y >> 16));
+      }
+      for (int x = nG + threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = 0;
+    } else {
+      for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? 4 * (int32_t)g[x] : 0;
+    }
+  } else {
+    const int32_t* g = Gt + (size_t)b * nG;
+    for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? 4 * g[x] : 0;
+  }
   __syncthreads();
   const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
   const uint32_t in = inst[b];

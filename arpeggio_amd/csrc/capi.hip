@@ -73,7 +73,7 @@ int hip_status(hipError_t e, const char* what) { return hip_check(e, what); }
 
 extern "C" const char* arp_last_error(void) { return g_err.c_str(); }
 
-extern "C" int arp_abi_version(void) { return 2; }
+extern "C" int arp_abi_version(void) { return 3; }
 
 extern "C" int arp_set_anneal_waves(int waves) {
   if (waves < 0 || waves > 16) return fail(ARP_E_ARG, "set_anneal_waves: %d outside 0..16", waves);
@@ -140,7 +140,7 @@ extern "C" int arp_anneal_chord(const int16_t* J, const int32_t* h, const int32_
 extern "C" int arp_front_end(const float* H, const float* y, const float* N0, float N0_scalar, int B, int Mr, int D,
                              int M, int K, float Q, int8_t* xg, int32_t* Gi, int32_t* hi, float* scale, float* Gf,
                              float* hf, int16_t* J, int32_t* hN, float* z, float* mu, float* Jf, float* hNf,
-                             uint16_t* feat, int feat_ld, void* stream) {
+                             uint16_t* feat, int feat_ld, int16_t* Gh, void* stream) {
   if (D < 1 || D > 128) return fail(ARP_E_RANGE, "front_end: D = %d outside 1..128", D);
   if (Mr < 1 || Mr > 128) return fail(ARP_E_RANGE, "front_end: Mr = %d outside 1..128", Mr);
   if (M != 4 && M != 16) return fail(ARP_E_ARG, "front_end: M = %d, expected 4 or 16", M);
@@ -151,7 +151,7 @@ extern "C" int arp_front_end(const float* H, const float* y, const float* N0, fl
   if (!H || !y) return fail(ARP_E_ARG, "front_end: null input");
   if (feat && feat_ld < arp::feat_count(D))
     return fail(ARP_E_ARG, "front_end: feat_ld = %d < %d features", feat_ld, arp::feat_count(D));
-  arp::FrontOut o{xg, Gi, hi, scale, Gf, hf, J, hN, z, mu, Jf, hNf, feat, feat_ld};
+  arp::FrontOut o{xg, Gi, hi, scale, Gf, hf, J, hN, z, mu, Jf, hNf, feat, feat_ld, Gh};
   return hip_check(arp::launch_front_end(H, y, N0, N0_scalar, B, Mr, D, M, K, Q, o, (hipStream_t)stream),
                    "front_end launch");
 }
@@ -346,7 +346,8 @@ extern "C" int arp_start_thresholds(const float* z, long ld, int M, int N, float
   return hip_check(arp::launch_start_thr(z, ld, M, N, inv_temp, thr, (hipStream_t)stream), "start_thr launch");
 }
 
-extern "C" int arp_anneal_arpeggio_list(const int32_t* Gt, const int32_t* h, const int32_t* T, int L, int q, int P,
+extern "C" int arp_anneal_arpeggio_list(const int32_t* Gt, const int16_t* Gh, const int32_t* h, const int32_t* T, int L,
+                                        int q, int P,
                                         int R, int D, int B, const uint32_t* inst, uint32_t k0, uint32_t k1, int start,
                                         const uint32_t* thr, int gate, int r0, const int32_t* list,
                                         const int32_t* count, int8_t* out_s, int64_t* out_de, int64_t* out_st,
@@ -358,12 +359,13 @@ extern "C" int arp_anneal_arpeggio_list(const int32_t* Gt, const int32_t* h, con
   if (start != 0 && start != 1) return fail(ARP_E_ARG, "arpeggio_list: start = %d", start);
   if (!list != !count) return fail(ARP_E_ARG, "arpeggio_list: list and count go together");
   if (B == 0) return ok();
-  if (!Gt || !h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio_list: null pointer");
+  if (!Gt == !Gh) return fail(ARP_E_ARG, "arpeggio_list: exactly one of Gt (int32) and Gh (int16) must be given");
+  if (!h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio_list: null pointer");
   const int waves = arpeggio_waves(R, D);
   if (!block_fits(waves, ARP_ARPEGGIO_MAX_THREADS))
     return fail(ARP_E_ARG, "arpeggio_list: %d anneals per workgroup (%d threads) exceed the kernel's bound of %d "
                 "threads", waves, ARP_WAVE * waves, ARP_ARPEGGIO_MAX_THREADS);
-  arp::AnnealExtra ex{thr, thr ? out_used : nullptr, list, count, gate ? 1 : 0, r0};
+  arp::AnnealExtra ex{thr, thr ? out_used : nullptr, list, count, gate ? 1 : 0, r0, nullptr, 0, Gh};
   return hip_check(arp::launch_arpeggio_ex(Gt, h, T, L, q, P, R, D, B, inst, k0, k1, thr ? 0 : start, 0, out_s,
                                            (long long*)out_de, (long long*)out_st, waves, ex, (hipStream_t)stream),
                    "arpeggio_list launch");

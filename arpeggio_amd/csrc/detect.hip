@@ -13,7 +13,7 @@ namespace arp {
 
 struct DetectWork {
   int8_t* xg;
-  int32_t* Gi;
+  int16_t* Gh;
   int32_t* hi;
   float* scale;
   uint16_t* feat;
@@ -46,7 +46,7 @@ static DetectWork carve(uint8_t* base, int B, int D, int R1, int R2, int feat_ld
   };
   const size_t b = (size_t)B, N = 2 * (size_t)D;
   w.xg = (int8_t*)take(b * D);
-  w.Gi = (int32_t*)take(b * D * D * 4);
+  w.Gh = (int16_t*)take(b * D * D * 2);
   w.hi = (int32_t*)take(b * D * 4);
   w.scale = (float*)take(b * 4);
   if (learned) {
@@ -112,8 +112,8 @@ int detect_chain(const float* H, const float* y, const float* N0, float N0_scala
     return set_error(ARP_E_ARG, "detect: anneals per workgroup exceed the kernel's bound");
   const DetectWork w = carve((uint8_t*)work, B, D, R1, R2, fld, learned);
   hipStream_t st = (hipStream_t)stream;
-  FrontOut o{w.xg, w.Gi, w.hi, w.scale, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-             nullptr, nullptr, learned ? w.feat : nullptr, learned ? fld : 0};
+  FrontOut o{w.xg, nullptr, w.hi, w.scale, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+             nullptr, nullptr, learned ? w.feat : nullptr, learned ? fld : 0, w.Gh};
   hipError_t e = launch_front_end(H, y, N0, N0_scalar, B, Mr, D, M, 1, 4096.f, o, st);
   if (e != hipSuccess) return hip_status(e, "detect: front end");
   if (learned) {
@@ -121,8 +121,8 @@ int detect_chain(const float* H, const float* y, const float* N0, float N0_scala
                                mlp->inv_temp, w.thr, st);
     if (e != hipSuccess) return hip_status(e, "detect: warm-start MLP");
   }
-  AnnealExtra ex1{w.thr, w.used1, nullptr, nullptr, learned && gate ? 1 : 0, 0};
-  e = launch_arpeggio_ex(w.Gi, w.hi, T1, L1, q, 2, R1, D, B, inst, k0, k1, 0, 0, w.spins1, w.de1, w.st1, waves1, ex1,
+  AnnealExtra ex1{w.thr, w.used1, nullptr, nullptr, learned && gate ? 1 : 0, 0, nullptr, 0, w.Gh};
+  e = launch_arpeggio_ex(nullptr, w.hi, T1, L1, q, 2, R1, D, B, inst, k0, k1, 0, 0, w.spins1, w.de1, w.st1, waves1, ex1,
                          st);
   if (e != hipSuccess) return hip_status(e, "detect: stage-1 anneal");
   e = launch_decode_ex(H, y, w.xg, w.spins1, B, R1, 2, D, Mr, M, 1, ltrue, lhat, biterr, nullptr,
@@ -131,8 +131,8 @@ int detect_chain(const float* H, const float* y, const float* N0, float N0_scala
   if (R2 == 0) return set_error(0, "");
   e = launch_certify(w.d1, N0, N0_scalar, thr_scale, B, w.list, w.count, st);
   if (e != hipSuccess) return hip_status(e, "detect: certificate");
-  AnnealExtra ex2{w.thr, w.used2, w.list, w.count, learned && gate ? 1 : 0, R1};
-  e = launch_arpeggio_ex(w.Gi, w.hi, T2, L2, q, 2, R2, D, B, inst, k0, k1, 0, 0, w.spins2, w.de2, w.st2, waves2, ex2,
+  AnnealExtra ex2{w.thr, w.used2, w.list, w.count, learned && gate ? 1 : 0, R1, nullptr, 0, w.Gh};
+  e = launch_arpeggio_ex(nullptr, w.hi, T2, L2, q, 2, R2, D, B, inst, k0, k1, 0, 0, w.spins2, w.de2, w.st2, waves2, ex2,
                          st);
   if (e != hipSuccess) return hip_status(e, "detect: stage-2 anneal");
   e = launch_decode_ex(H, y, w.xg, w.spins2, B, R2, 2, D, Mr, M, 1, ltrue, lhat, biterr, nullptr,

@@ -226,6 +226,7 @@ __global__ __launch_bounds__(64) void front_end_w64_kernel(const float* __restri
       const float gs = Gs[i * ld + lane] / scale;
       const int32_t gi = (int32_t)rintf(gs * Q);
       if (o.Gi) o.Gi[(size_t)b * D * D + i * D + lane] = gi;
+      if (o.Gh) o.Gh[(size_t)b * D * D + i * D + lane] = (int16_t)gi;
       if (o.Gf) o.Gf[(size_t)b * D * D + i * D + lane] = gs;
       if (o.J || o.Jf)
         for (int pr = 0; pr < P; ++pr)
@@ -479,6 +480,7 @@ __global__ __launch_bounds__(128) void front_end_w128_kernel(const float* __rest
       const float gs = Gs[i * ld + j] / scale;
       const int32_t gi = (int32_t)rintf(gs * Q);
       if (o.Gi) o.Gi[(size_t)b * D * D + (size_t)i * D + j] = gi;
+      if (o.Gh) o.Gh[(size_t)b * D * D + (size_t)i * D + j] = (int16_t)gi;
       if (o.Gf) o.Gf[(size_t)b * D * D + (size_t)i * D + j] = gs;
       if (o.J || o.Jf)
         for (int pr = 0; pr < P; ++pr)

@@ -20,6 +20,7 @@ struct FrontOut {
   uint16_t* feat;  // (B, feat_ld) bf16 learned-start features: H^T y / g, G_jj / g - 1, x_g / lmax, z/mu - x_g,
                    // (N0/2) / g, zero padding; g = mean_j G_jj
   int feat_ld;
+  int16_t* Gh;     // (B, D, D) fixed-point Gram as int16 (|G_int| <= Q <= 2^14 fits): Gi at half the bytes
 };
 
 // One class of a mixed batch for the persistent Arpeggio launch: frames of one size D with their own buffers (the

@@ -35,7 +35,7 @@ def _n0_arg(N0, B):
 def front_end(H, y, N0, M, K=1, Q=QSCALE, want=('xg', 'Gi', 'hi', 'scale')):
     """H (B, Mr, D), y (B, Mr) device float tensors; N0 scalar or (B,). Returns a dict with the requested outputs:
     xg (B, D) int8, Gi (B, D, D) int32, hi (B, D) int32, scale (B,), Gf, hf (float G/scale, h/scale), J (B, N, N)
-    int16 and hN (B, N) int32 (N = 2K D), z, mu (B, D)."""
+    int16 and hN (B, N) int32 (N = 2K D), z, mu (B, D), Gh (B, D, D) int16 (Gi at half the bytes)."""
     H, y = _f32(H), _f32(y)
     B, Mr, D = H.shape
     N = 2 * K * D
@@ -43,13 +43,15 @@ def front_end(H, y, N0, M, K=1, Q=QSCALE, want=('xg', 'Gi', 'hi', 'scale')):
     shapes = dict(xg=((B, D), torch.int8), Gi=((B, D, D), torch.int32), hi=((B, D), torch.int32),
                   scale=((B,), torch.float32), Gf=((B, D, D), torch.float32), hf=((B, D), torch.float32),
                   J=((B, N, N), torch.int16), hN=((B, N), torch.int32), z=((B, D), torch.float32),
-                  mu=((B, D), torch.float32), Jf=((B, N, N), torch.float32), hNf=((B, N), torch.float32))
+                  mu=((B, D), torch.float32), Jf=((B, N, N), torch.float32), hNf=((B, N), torch.float32),
+                  Gh=((B, D, D), torch.int16))
     out = {k: torch.empty(*shapes[k][0], dtype=shapes[k][1], device=dev) for k in want}
     n0t, n0s = _n0_arg(N0, B)
     order = ('xg', 'Gi', 'hi', 'scale', 'Gf', 'hf', 'J', 'hN', 'z', 'mu', 'Jf', 'hNf')
     L = _lib.load()
     _lib.check(L.arp_front_end(_lib.ptr(H), _lib.ptr(y), _lib.ptr(n0t), n0s, B, Mr, D, int(M), int(K), float(Q),
-                               *[_lib.ptr(out.get(k)) for k in order], None, 0, _lib.stream_ptr()), 'front_end')
+                               *[_lib.ptr(out.get(k)) for k in order], None, 0, _lib.ptr(out.get('Gh')),
+                               _lib.stream_ptr()), 'front_end')
     return out
 
 
@@ -104,7 +106,8 @@ class DetectPipeline:
             schedule_check(T, qi, self.P, D, Q)
             self.T = torch.tensor(T, dtype=torch.int32, device=dev)
             self.q = qi
-            self.Gi = torch.empty(B, D, D, dtype=torch.int32, device=dev)
+            # first accept reads the Gram as int16 (the front end's Gh); the parallel pass takes the int32 form
+            self.Gi = torch.empty(B, D, D, dtype=torch.int16 if mode == 'arpeggio' else torch.int32, device=dev)
             self.hi = torch.empty(B, D, dtype=torch.int32, device=dev)
             self.inst = ((torch.arange(B, dtype=torch.int64) + inst0) & MASK).to(torch.int32).to(dev)
             self.spins = torch.empty(B, R, N, dtype=torch.int8, device=dev)
@@ -169,35 +172,52 @@ class DetectPipeline:
         B, Mr, D, M, R, P = self.B, self.Mr, self.D, self.M, self.R, self.P
         n0t, n0s = _n0_arg(N0, B)
         P_ = _lib.ptr
+        g16 = self.policy == 'arpeggio' and self.mode == 'arpeggio'
         if self.policy == 'arpeggio' and self.start == 'learned':
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
-                                       P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None,
-                                       None, None, P_(self.feat), self.feat.shape[1], s), 'front_end')
+                                       None if g16 else P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None,
+                                       None, None, None, None, None, P_(self.feat), self.feat.shape[1],
+                                       P_(self.Gi) if g16 else None, s), 'front_end')
             stage(1)
             self.model.thresholds(self.feat, self.inv_temp, out=self.thr, stream=stream)
             stage(2)
             mark(0)
-            _lib.check(L.arp_anneal_arpeggio_ws(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D,
-                                                B, P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
-                                                P_(self.thr), 1 if self.gate else 0, P_(self.spins), P_(self.de),
-                                                P_(self.st), P_(self.used), s), 'arpeggio_ws')
+            if g16:
+                _lib.check(L.arp_anneal_arpeggio_list(None, P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(),
+                                                      self.q, P, R, D, B, P_(self.inst), self.key & MASK,
+                                                      (self.key >> 32) & MASK, 0, P_(self.thr), 1 if self.gate else 0,
+                                                      0, None, None, P_(self.spins), P_(self.de), P_(self.st),
+                                                      P_(self.used), s), 'arpeggio_list')
+            else:
+                _lib.check(L.arp_anneal_arpeggio_ws(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R,
+                                                    D, B, P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
+                                                    P_(self.thr), 1 if self.gate else 0, P_(self.spins), P_(self.de),
+                                                    P_(self.st), P_(self.used), s), 'arpeggio_ws')
             mark(1)
         elif self.policy == 'arpeggio':
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg),
-                                       P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None, None, None, None,
-                                       None, None, None, 0, s), 'front_end')
+                                       None if g16 else P_(self.Gi), P_(self.hi), P_(self.scale), None, None, None,
+                                       None, None, None, None, None, None, 0, P_(self.Gi) if g16 else None, s),
+                       'front_end')
             stage(1)
             stage(2)
             mark(0)
-            _lib.check(L.arp_anneal_arpeggio(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D, B,
-                                             P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
-                                             0 if self.start == 'rand' else 1, 0 if self.mode == 'arpeggio' else 1,
-                                             P_(self.spins), P_(self.de), P_(self.st), s), 'arpeggio')
+            if g16:
+                _lib.check(L.arp_anneal_arpeggio_list(None, P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(),
+                                                      self.q, P, R, D, B, P_(self.inst), self.key & MASK,
+                                                      (self.key >> 32) & MASK, 0 if self.start == 'rand' else 1, None,
+                                                      0, 0, None, None, P_(self.spins), P_(self.de), P_(self.st), None,
+                                                      s), 'arpeggio_list')
+            else:
+                _lib.check(L.arp_anneal_arpeggio(P_(self.Gi), P_(self.hi), P_(self.T), self.T.numel(), self.q, P, R, D,
+                                                 B, P_(self.inst), self.key & MASK, (self.key >> 32) & MASK,
+                                                 0 if self.start == 'rand' else 1, 1, P_(self.spins), P_(self.de),
+                                                 P_(self.st), s), 'arpeggio')
             mark(1)
         else:
             _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg), None,
                                        None, P_(self.scale), None, None, P_(self.J), P_(self.hN), None, None, None,
-                                       None, None, 0, s), 'front_end')
+                                       None, None, 0, None, s), 'front_end')
             stage(1)
             stage(2)
             mark(0)
@@ -255,7 +275,7 @@ class AdaptivePipeline:
             sched.append(torch.tensor(T, dtype=torch.int32, device=dev))
         self.T1t, self.T2t, self.q = sched[0], sched[1], qi
         self.xg = torch.empty(B, D, dtype=torch.int8, device=dev)
-        self.Gi = torch.empty(B, D, D, dtype=torch.int32, device=dev)
+        self.Gi = torch.empty(B, D, D, dtype=torch.int16, device=dev)   # the front end's int16 Gram (Gh)
         self.hi = torch.empty(B, D, dtype=torch.int32, device=dev)
         self.scale = torch.empty(B, dtype=torch.float32, device=dev)
         self.inst = ((torch.arange(B, dtype=torch.int64) + inst0) & MASK).to(torch.int32).to(dev)
@@ -306,10 +326,10 @@ class AdaptivePipeline:
         n0t, n0s = _n0_arg(N0, B)
         P_ = _lib.ptr
         learned = self.thr is not None
-        _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg), P_(self.Gi),
+        _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, self.K, self.Q, P_(self.xg), None,
                                    P_(self.hi), P_(self.scale), None, None, None, None, None, None, None, None,
-                                   P_(self.feat) if learned else None, self.feat.shape[1] if learned else 0, s),
-                   'front_end')
+                                   P_(self.feat) if learned else None, self.feat.shape[1] if learned else 0,
+                                   P_(self.Gi), s), 'front_end')
         mark(1)
         if learned:
             self.model.thresholds(self.feat, self.inv_temp, out=self.thr, stream=stream)
@@ -318,8 +338,8 @@ class AdaptivePipeline:
             events[0].record(ts)
         k0, k1 = self.key & MASK, (self.key >> 32) & MASK
         gate = 1 if learned and self.gate else 0
-        _lib.check(L.arp_anneal_arpeggio_list(P_(self.Gi), P_(self.hi), P_(self.T1t), self.T1t.numel(), self.q, P,
-                                              self.R1, D, B, P_(self.inst), k0, k1, 0, P_(self.thr), gate, 0, None,
+        _lib.check(L.arp_anneal_arpeggio_list(None, P_(self.Gi), P_(self.hi), P_(self.T1t), self.T1t.numel(), self.q,
+                                              P, self.R1, D, B, P_(self.inst), k0, k1, 0, P_(self.thr), gate, 0, None,
                                               None, P_(self.spins1), P_(self.de1), P_(self.st1),
                                               P_(self.used1) if learned else None, s), 'anneal stage 1')
         mark(3)
@@ -332,8 +352,8 @@ class AdaptivePipeline:
         _lib.check(L.arp_certify(P_(self.d1), P_(n0t), n0s, self.thr_scale, B, P_(self.list), P_(self.count), s),
                    'certify')
         mark(5)
-        _lib.check(L.arp_anneal_arpeggio_list(P_(self.Gi), P_(self.hi), P_(self.T2t), self.T2t.numel(), self.q, P,
-                                              self.R2, D, B, P_(self.inst), k0, k1, 0, P_(self.thr), gate, self.R1,
+        _lib.check(L.arp_anneal_arpeggio_list(None, P_(self.Gi), P_(self.hi), P_(self.T2t), self.T2t.numel(), self.q,
+                                              P, self.R2, D, B, P_(self.inst), k0, k1, 0, P_(self.thr), gate, self.R1,
                                               P_(self.list), P_(self.count), P_(self.spins2), P_(self.de2),
                                               P_(self.st2), P_(self.used2) if learned else None, s), 'anneal stage 2')
         mark(6)

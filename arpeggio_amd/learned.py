@@ -189,7 +189,7 @@ def features(H, y, N0, M, K=1, Q=4096.0):
     p = _lib.ptr
     _lib.check(_lib.load().arp_front_end(p(H), p(y), p(n0t), n0s, B, Mr, D, int(M), int(K), float(Q), p(out['xg']),
                                          p(out['Gi']), p(out['hi']), p(out['scale']), None, None, None, None, None,
-                                         None, None, None, p(out['feat']), F, _lib.stream_ptr()), 'front_end')
+                                         None, None, None, p(out['feat']), F, None, _lib.stream_ptr()), 'front_end')
     return out
 
 

@@ -97,7 +97,8 @@ class MDIDetector:
             x = torch.empty(B, D, dtype=torch.int8, device='cuda')
             if g == 'mmse':
                 _lib.check(L.arp_front_end(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, 1, self.Q, P_(x), None, None, None,
-                                           None, None, None, None, None, None, None, None, None, 0, s), 'front_end')
+                                           None, None, None, None, None, None, None, None, None, 0, None, s),
+                           'front_end')
             else:
                 _lib.check(L.arp_mmse_sic(P_(H), P_(y), P_(n0t), n0s, B, Mr, D, M, 0, P_(x), s), 'mmse_sic')
             out[g] = x
@@ -129,8 +130,8 @@ class MDIDetector:
                                                na, D, n, P_(self.inst), key & MASK, (key >> 32) & MASK, 1, P_(idx),
                                                P_(self.count), P_(spins), P_(de), P_(st), s), 'Chord (MDI)')
         else:
-            _lib.check(L.arp_anneal_arpeggio_list(P_(self.Gi), P_(self.hi), P_(T), T.numel(), self.q, 2 * K, na, D, n,
-                                                  P_(self.inst), key & MASK, (key >> 32) & MASK,
+            _lib.check(L.arp_anneal_arpeggio_list(P_(self.Gi), None, P_(self.hi), P_(T), T.numel(), self.q, 2 * K, na, D,
+                                                  n, P_(self.inst), key & MASK, (key >> 32) & MASK,
                                                   0 if self.pk['start'] == 'rand' else 1, None, 0, 0, P_(idx),
                                                   P_(self.count), P_(spins), P_(de), P_(st), None, s), 'anneal (MDI)')
         return spins, st

@@ -87,8 +87,9 @@ int arp_anneal_chord(const int16_t *J, const int32_t *h, const int32_t *h0, cons
                      int q, int S, int P, int R, int N, int B, uint32_t k0, uint32_t k1, int inst0, int mode,
                      int32_t *out_e, int8_t *out_s, uint64_t *stats, void *stream);
 
-/* Fused front end, one workgroup per frame: LMMSE hard decision (in-register symmetric-sweep inversion of
- * H^T H + (N0/2)/E[l^2] I; one wave per frame for D <= 64, two for D <= 128),
+/* Fused front end, one workgroup per frame: LMMSE hard decision (for D, Mr <= 64 the inverse of
+ * H^T H + (N0/2)/E[l^2] I by a four-pivot block sweep on the matrix cores, see arp_set_front_end_form; otherwise an
+ * in-register symmetric sweep, one wave per frame for D <= 64, two for D <= 128),
  * delta-Ising build around it (dDI, 2K copies) on the matrix cores, and the fixed-point inputs of the annealers.
  * Replaces, on the kernel path, mimo.mmse / mmse_hard / _mu / quantize_levels (arpeggio/mimo/mimo.py:86-109, 176-182),
  * mimo.ising_delta_at (mimo.py:506-523), the J/h rounding of chordfx.to_fixed (arpeggio/mimo/chordfx.py:60-76) and
@@ -101,10 +102,11 @@ int arp_anneal_chord(const int16_t *J, const int32_t *h, const int32_t *h0, cons
  * h/scale; J (B, N, N) int16 and hN (B, N) int32, the expanded Chord inputs (N = 2 K D <= 256); z, mu (B, D);
  * Jf (B, N, N), hNf (B, N) float32, the expanded inputs of the FP32 Chord anneal; feat (B, feat_ld) bf16 (bit
  * pattern as uint16), the learned-start features [H^T y / g, G_jj / g - 1, x_g / lmax, z/mu - x_g (D each),
- * (N0/2) / g, zeros], g = mean_j G_jj, feat_ld >= 4 D + 1 (no reference counterpart). */
+ * (N0/2) / g, zeros], g = mean_j G_jj, feat_ld >= 4 D + 1 (no reference counterpart); Gh (B, D, D) int16, the
+ * same integers as Gi at half the bytes (|G_int| <= Q), which arp_anneal_arpeggio_list takes in place of Gt. */
 int arp_front_end(const float *H, const float *y, const float *N0, float N0_scalar, int B, int Mr, int D, int M, int K,
                   float Q, int8_t *xg, int32_t *Gi, int32_t *hi, float *scale, float *Gf, float *hf, int16_t *J,
-                  int32_t *hN, float *z, float *mu, float *Jf, float *hNf, uint16_t *feat, int feat_ld,
+                  int32_t *hN, float *z, float *mu, float *Jf, float *hNf, uint16_t *feat, int feat_ld, int16_t *Gh,
                   void *stream);
 
 /* Decode and select: candidate levels from anneal spins (delta = 1: clip(x_g + sum of the P copies); delta = 0: RI
@@ -176,9 +178,11 @@ int arp_anneal_arpeggio_batched(const arp_batch_class *classes, int n_classes, i
  * launched (the list's capacity). list = count = NULL: every frame b < B in its own slot (then r0 = 0 equals
  * arp_anneal_arpeggio / arp_anneal_arpeggio_ws). thr = NULL: the start `start` (0 rand, 1 bal); else the learned start
  * with the energy gate `gate` (as arp_anneal_arpeggio_ws). Anneal r of a frame is the same stream in every launch, so
- * r0 = R1 continues a frame's R1 anneals of an earlier launch with new ones. Mode: first accept. */
-int arp_anneal_arpeggio_list(const int32_t *Gt, const int32_t *h, const int32_t *T, int L, int q, int P, int R, int D,
-                             int B, const uint32_t *inst, uint32_t k0, uint32_t k1, int start, const uint32_t *thr,
+ * r0 = R1 continues a frame's R1 anneals of an earlier launch with new ones. Mode: first accept. The Gram comes as
+ * Gt (int32) or as Gh (int16, the front end's Gh output): exactly one of the two is given. */
+int arp_anneal_arpeggio_list(const int32_t *Gt, const int16_t *Gh, const int32_t *h, const int32_t *T, int L, int q,
+                             int P, int R, int D, int B, const uint32_t *inst, uint32_t k0, uint32_t k1, int start,
+                             const uint32_t *thr,
                              int gate, int r0, const int32_t *list, const int32_t *count, int8_t *out_s,
                              int64_t *out_de, int64_t *out_st, int8_t *out_used, void *stream);
 

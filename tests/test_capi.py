@@ -87,7 +87,7 @@ def test_detect_workspace_and_argument_errors_without_gpu():
     from arpeggio_amd import _lib
     L = _lib.load()
     n = L.arp_detect_workspace_bytes(1000, 64, 4, 16, 1)
-    assert n >= 1000 * (64 * 64 * 4 + 20 * 128) and n % 256 == 0
+    assert n >= 1000 * (64 * 64 * 2 + 20 * 128) and n % 256 == 0   # int16 Gram, spins of 20 anneals
     assert L.arp_detect_workspace_bytes(1000, 64, 4, 0, 0) < n
     assert L.arp_detect_workspace_bytes(10, 200, 4, 16, 0) == 0
     buf = ct.create_string_buffer(64)
@@ -110,3 +110,17 @@ def test_chord_lane_argument_errors_without_gpu():
     assert L.arp_anneal_chord_lane(*args(2, 129, 1)) == -3
     assert L.arp_anneal_chord_lane(*args(2, 16, 2)) == -1 and b'rng' in L.arp_last_error()
     assert L.arp_anneal_chord_lane(*args(2, 16, 1)) == -1 and b'null' in L.arp_last_error()
+
+
+def test_list_anneal_gram_forms_without_gpu():
+    """arp_anneal_arpeggio_list takes the Gram as int32 (Gt) or int16 (Gh), exactly one of them."""
+    import ctypes
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    one = ctypes.c_int(0)
+    args = lambda gt, gh: (gt, gh, None, None, 8, 0, 2, 2, 16, 1, None, 0, 0, 0, None, 0, 0, None, None, None, None,
+                           None, None, None)
+    p = ctypes.addressof(one)
+    for gt, gh in ((None, None), (p, p)):
+        assert L.arp_anneal_arpeggio_list(*args(gt, gh)) == -1 and b'exactly one' in L.arp_last_error()
+    assert L.arp_abi_version() == 3

@@ -23,15 +23,28 @@ def near_half(x, tol=1e-3):
     return np.abs(np.abs(x - np.floor(x)) - 0.5) < tol
 
 
+@pytest.fixture
+def fe_form():
+    """Selects a front-end form for one test and restores the default afterwards."""
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    yield lambda f: _lib.check(L.arp_set_front_end_form(f), 'set_front_end_form')
+    L.arp_set_front_end_form(0)
+
+
+@pytest.mark.parametrize('form', [0, 1])
 @pytest.mark.parametrize('nt,M,snr,K', [(16, 16, 16.0, 1), (32, 16, 20.0, 1), (16, 4, 8.0, 1), (64, 16, 22.0, 1),
-                                        (16, 16, 14.0, 2), (8, 16, 12.0, 3)])
-def test_front_end_vs_oracle(orc, nt, M, snr, K):
+                                        (16, 16, 14.0, 2), (8, 16, 12.0, 3), (31, 16, 20.0, 1), (24, 4, 10.0, 1)])
+def test_front_end_vs_oracle(orc, fe_form, nt, M, snr, K, form):
+    """Both front-end forms (0: matrix-core block sweep for D, Mr <= 64; 1: register sweep) against the oracle; the
+    int16 Gram equals the int32 one; the expanded Chord inputs (register-sweep kernels) expand the same integers."""
     from arpeggio_amd.detector import front_end
-    say('front_end', nt, M, snr, K)
+    say('front_end', nt, M, snr, K, 'form', form)
+    fe_form(form)
     b = frames(nt, M, snr, 64)
-    want = ('xg', 'Gi', 'hi', 'scale', 'Gf', 'hf', 'z', 'mu') + (('J', 'hN') if 2 * K * 2 * nt <= 256 else ())
-    o = front_end(b['H'], b['y'], b['N0'], M, K, want=want)
+    o = front_end(b['H'], b['y'], b['N0'], M, K, want=('xg', 'Gi', 'hi', 'scale', 'Gf', 'hf', 'z', 'mu', 'Gh'))
     torch.cuda.synchronize()
+    assert (o['Gh'].cpu().numpy().astype(np.int32) == o['Gi'].cpu().numpy()).all()
     H, y = b['H'].cpu().numpy(), b['y'].cpu().numpy()
     z, Ainv, _ = orc.mmse(H, y, b['N0'], M)
     assert np.allclose(o['z'].cpu().numpy(), z, rtol=1e-3, atol=1e-3)
@@ -56,9 +69,11 @@ def test_front_end_vs_oracle(orc, nt, M, snr, K):
     Gq, hq = I['G'] * 4096, I['hD'] * 4096
     assert ((Gi == np.rint(Gq)) | near_half(Gq, 0.05)).all()
     assert ((hi == np.rint(hq)) | near_half(hq, 0.05)).all()
-    if 'J' in o:
-        J, hN = orc.expand(Gi.astype(np.int64), hi, K)
-        assert (o['J'].cpu().numpy() == J).all() and (o['hN'].cpu().numpy() == hN).all()
+    if 2 * K * 2 * nt <= 256:
+        oj = front_end(b['H'], b['y'], b['N0'], M, K, want=('Gi', 'hi', 'J', 'hN'))
+        torch.cuda.synchronize()
+        J, hN = orc.expand(oj['Gi'].cpu().numpy().astype(np.int64), oj['hi'].cpu().numpy(), K)
+        assert (oj['J'].cpu().numpy() == J).all() and (oj['hN'].cpu().numpy() == hN).all()
 
 
 def test_front_end_golden(golden):

@@ -191,3 +191,41 @@ def test_learned_pipeline_on_side_stream():
             torch.cuda.synchronize()
         outs.append((pipe.spins.clone(), pipe.thr.clone(), pipe.lhat.clone(), pipe.biterr.clone()))
     assert all(torch.equal(a, c) for a, c in zip(*outs))
+
+
+@pytest.mark.parametrize('start,learned', [('rand', False), ('bal', False), ('rand', True)])
+def test_list_anneal_int16_gram_equals_int32(start, learned):
+    """arp_anneal_arpeggio_list on the front end's int16 Gram (Gh) gives exactly the anneals of the int32 Gram (Gt),
+    with and without a frame list and the learned-start thresholds."""
+    from arpeggio_amd import _lib, frames
+    from arpeggio_amd.detector import arpeggio_schedule, front_end
+    say('int16 gram', start, learned)
+    B, nt, R = 96, 32, 6
+    D = 2 * nt
+    b = frames.draw(77, 20.0, 0, B, nt, 16, dev='cuda')
+    o = front_end(b['H'], b['y'], b['N0'], 16, 1, want=('Gi', 'hi', 'Gh'))
+    T, q = arpeggio_schedule(12, 0.1, 0.1, 0.0, 1)
+    Tt = torch.tensor(T, dtype=torch.int32, device='cuda')
+    inst = torch.arange(B, dtype=torch.int32, device='cuda')
+    thr = torch.randint(0, 2 ** 31, (B, 2 * D), dtype=torch.int64, device='cuda').to(torch.int32) if learned else None
+    lst = torch.tensor(sorted(np.random.default_rng(3).choice(B, 40, replace=False)), dtype=torch.int32,
+                       device='cuda')
+    cnt = torch.tensor([40], dtype=torch.int32, device='cuda')
+    L = _lib.load()
+    outs = []
+    for gt, gh in ((o['Gi'], None), (None, o['Gh'])):
+        for ls, cn in ((None, None), (lst, cnt)):
+            s = torch.empty(B, R, 2 * D, dtype=torch.int8, device='cuda')
+            de = torch.empty(B, R, dtype=torch.int64, device='cuda')
+            st = torch.empty(B, R, 2, dtype=torch.int64, device='cuda')
+            used = torch.empty(B, R, dtype=torch.int8, device='cuda')
+            _lib.check(L.arp_anneal_arpeggio_list(_lib.ptr(gt), _lib.ptr(gh), _lib.ptr(o['hi']), _lib.ptr(Tt),
+                                                  len(T), q, 2, R, D, B, _lib.ptr(inst), 0x1234, 0x5678,
+                                                  0 if start == 'rand' else 1, _lib.ptr(thr), 1, 0, _lib.ptr(ls),
+                                                  _lib.ptr(cn), _lib.ptr(s), _lib.ptr(de), _lib.ptr(st),
+                                                  _lib.ptr(used) if learned else None, _lib.stream_ptr()), 'list')
+            torch.cuda.synchronize()
+            n = B if ls is None else 40
+            outs.append((s[:n].cpu().numpy(), de[:n].cpu().numpy(), st[:n].cpu().numpy()))
+    for a, c in ((outs[0], outs[2]), (outs[1], outs[3])):
+        assert all((x == y).all() for x, y in zip(a, c))
