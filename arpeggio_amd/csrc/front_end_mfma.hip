@@ -3,8 +3,8 @@
 // The same outputs as the register-sweep front end (front_end.hip) from a different schedule. One wave per frame:
 //   1. G = H^T H in v_mfma_f32_16x16x4f32 accumulators (the upper 16x16 tiles of the coordinates in tile order,
 //      position 16 I + m = coordinate nT m + I, so that a lane's operand columns are adjacent in memory), H read once
-//      from global memory straight into the operands; c = H^T y by the VALU beside it. A copy of the tiles waits in
-//      LDS (10 KiB at D = 64, in the accumulator layout) for the outputs.
+//      from global memory straight into the operands; c = H^T y by the VALU beside it. A copy of G waits in LDS
+//      (16.3 KiB at D = 64: all 16 tiles, the lower ones as transposes, each column-major) for the outputs.
 //   2. A = G + sigma I (identity on the padding) is swept in place in the accumulators, four pivots at a time. For the
 //      pivot set S = {k0 .. k0+3} the block sweep is
 //          A_SS -> -A_SS^-1,  A_RS -> A_RS A_SS^-1,  A_SR -> A_SS^-1 A_SR,  A_RR -> A_RR - A_RS A_SS^-1 A_SR,
@@ -15,13 +15,15 @@
 //      1 KiB LDS buffer, and the 4x4 inverse of A_SS is swept in registers by every lane. The right-hand side c rides
 //      along as an extra column (c <- c - V A_SS^-1 c_S), so it ends as z = A^-1 c.
 //   3. mu_i = 1 - sigma (A^-1)_ii, x_g = nearest level of z / mu (ties to the lower level).
-//   4. h = 2 H^T (y - H x_g) (mimo.ising_delta_at) as 2 (c - G x_g) from the Gram tiles, with cross-lane sums;
+//   4. h = 2 H^T (y - H x_g) (mimo.ising_delta_at) as 2 (c - G x_g), one row of the LDS copy per lane;
 //      scale = max(max|h|, max|G|), clamped at 1e-12; G_int = rne(G / scale * Q), h_int = rne(h / scale * Q),
-//      G written row-major in runs of four entries gathered from the LDS copy.
+//      G written row-major: 16-byte loads of four rows' entries from the LDS copy, whole rows per store instruction.
 // Reference semantics: mimo.mmse / _mu / quantize_levels (arpeggio/mimo/mimo.py:86-109, 176-182), mimo.ising_delta_at
 // (mimo.py:506-523), chordfx.to_fixed (arpeggio/mimo/chordfx.py:60-76), chord_ref.compress
 // (arpeggio/sim/x3model/chord_ref.py:47-59). The expanded Chord inputs (J, Jf) are the register-sweep kernel's.
 #include "kernels.h"
+
+#include <cmath>
 
 namespace arp {
 
@@ -52,39 +54,13 @@ __device__ constexpr int tile(int I, int J) {
   return I * nT - I * (I - 1) / 2 + (J - I);
 }
 
-// Reduce-scatter over the 16 lanes of a group (lane bits 0..3): v holds NP partial sums per lane (NP a power of two,
-// 4..16); afterwards v[0] is the sum over the 16 lanes of entry k(col), where step m (8, 4, 2, 1) keeps the upper half
-// on lanes with bit m set while more than one entry is left.
-template <int NP>
-__device__ __forceinline__ float reduce_scatter16(float (&v)[NP], int col) {
-  int n = NP;
-#pragma unroll
-  for (int m = 8; m >= 1; m >>= 1) {
-    if (n > 1) {
-      const bool up = (col & m) != 0;
-      const int half = n / 2;
-#pragma unroll
-      for (int i = 0; i < NP / 2; ++i)
-        if (i < half) {
-          const float send = up ? v[i] : v[i + half];
-          const float keep = up ? v[i + half] : v[i];
-          v[i] = keep + __shfl_xor(send, m);
-        }
-      n = half;
-    } else {
-      v[0] += __shfl_xor(v[0], m);
-    }
-  }
-  return v[0];
-}
-
 template <int DP>
 __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restrict__ H, const float* __restrict__ y,
                                                             const float* __restrict__ N0v, float N0s, int Mr, int D,
                                                             int M, int K, float Q, int vec, FrontOut o) {
   constexpr int nT = DP / 16, nU = nT * (nT + 1) / 2;
-  constexpr int NP = 4 * nT == 12 ? 16 : 4 * nT;        // row partial sums per lane in the G x reduction
-  __shared__ __align__(16) floatx4 Gs[nU * 64];         // the Gram tiles as each lane holds them
+  constexpr int TS = 260;                               // tile stride in words: 16 x 16 entries + 4 (bank offset)
+  __shared__ __align__(16) float Gt[nT * nT * TS];      // the full Gram, tile by tile, each tile column-major
   __shared__ __align__(16) float pan[2][DP * 4];        // panel rows S: pan[p * 4 + t] = A[k0 + t][p]
   __shared__ __align__(16) float bc[64];                // x_g in tile order
   const int b = blockIdx.x, lane = threadIdx.x, col = lane & 15, grp = lane >> 4;
@@ -149,9 +125,20 @@ __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restr
     c[I] = v;
     c0[I] = v;
   }
-  // G waits in LDS, in the accumulator layout, until the outputs
+  // G waits in LDS for the outputs as the full tile set, the lower tiles the transposes of the upper ones: tile (I, J)
+  // entry (r, c) = G[nT r + I][nT c + J] at Gt[(I nT + J) TS + 16 c + r]. Lane (grp, col) holds entries (4 grp + v,
+  // col) of upper tile (I, J): one 16-byte store, and four single stores of the mirror entries of tile (J, I).
 #pragma unroll
-  for (int u = 0; u < nU; ++u) Gs[u * 64 + lane] = T[u];
+  for (int I = 0; I < nT; ++I)
+#pragma unroll
+    for (int J = I; J < nT; ++J) {
+      const floatx4 t = T[tile<nT>(I, J)];
+      *reinterpret_cast<floatx4*>(Gt + (I * nT + J) * TS + 16 * col + 4 * grp) = t;
+      if (I < J) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Gt[(J * nT + I) * TS + 16 * (4 * grp + v) + col] = t[v];
+      }
+    }
   const float el2 = M == 4 ? 1.f : 5.f;
   const float n0 = N0v ? N0v[b] : N0s;
   const float sigma = (n0 / 2.f) / el2;
@@ -265,7 +252,8 @@ __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restr
       dgi = 2.f - td;
     }
   }
-  const float gii = valid ? Gs[(Ii * nT - Ii * (Ii - 1) / 2) * 64 + src][mi & 3] : 0.f;
+  float gii = 0.f;
+  if (valid) gii = Gt[(Ii * nT + Ii) * TS + 17 * mi];
   const float mu = 1.f - sigma * dgi;
   const float vq = z / mu;
   const float lmax = M == 4 ? 1.f : 3.f;
@@ -291,53 +279,29 @@ __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restr
     for (int x = 4 * D + lane; x < o.feat_ld; x += 64) f[x] = bf16_of(x == 4 * D ? sigma / gbar : 0.f);
   }
 
-  // 4. h = 2 (c - G x_g) in tile order. Tile (I, J) entry (16 I + 4 grp + v, 16 J + col) adds G x at 16 J + col to
-  // row 16 I + 4 grp + v (summed over the 16 lanes of the group by the reduce-scatter) and, for I < J, G x at
-  // 16 I + 4 grp + v to row 16 J + col (summed over the four groups).
+  // 4. h = 2 H^T (y - H x_g) (mimo.ising_delta_at) as 2 (c - G x_g): lane i = nT m + I takes the dot product of row
+  // i with x_g, reading the row from tiles (I, J), entries (m, c) at 16 c + m (the lanes of one I read 16 consecutive
+  // words and the nT values of I sit 16 banks apart), and x_g in tile order as broadcasts; the row's largest |G|
+  // rides along. scale = max(max|h|, max|G|), clamped at 1e-12.
   if (lane < DP) bc[16 * Ii + mi] = l;
   __syncthreads();
-  float ra[NP];
+  float gx = 0.f, gm = 0.f;
+  if (lane < DP) {
+    const float* grow = Gt + Ii * nT * TS + mi;
 #pragma unroll
-  for (int k = 0; k < NP; ++k) ra[k] = 0.f;
-  float rb[nT];
+    for (int J = 0; J < nT; ++J)
 #pragma unroll
-  for (int J = 0; J < nT; ++J) rb[J] = 0.f;
-  float gm = 0.f;
+      for (int c4 = 0; c4 < 16; c4 += 4) {
+        const floatx4 xv = *reinterpret_cast<const floatx4*>(bc + 16 * J + c4);
 #pragma unroll
-  for (int I = 0; I < nT; ++I) {
-    const floatx4 xi = *reinterpret_cast<const floatx4*>(bc + 16 * I + 4 * grp);
-#pragma unroll
-    for (int J = I; J < nT; ++J) {
-      const floatx4 g = Gs[tile<nT>(I, J) * 64 + lane];
-      const float xj = bc[16 * J + col];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        ra[4 * I + v] = __builtin_fmaf(g[v], xj, ra[4 * I + v]);
-        gm = fmaxf(gm, fabsf(g[v]));  // padding entries are zero
+        for (int e = 0; e < 4; ++e) {
+          const float g = grow[J * TS + 16 * (c4 + e)];
+          gx = __builtin_fmaf(g, xv[e], gx);
+          gm = fmaxf(gm, fabsf(g));   // padding entries are zero
+        }
       }
-      if (I < J)
-        rb[J] = __builtin_fmaf(g[3], xi[3], __builtin_fmaf(g[2], xi[2], __builtin_fmaf(g[1], xi[1],
-                                                                                       __builtin_fmaf(g[0], xi[0], rb[J]))));
-    }
   }
-#pragma unroll
-  for (int J = 0; J < nT; ++J) {
-    rb[J] += __shfl_xor(rb[J], 16);
-    rb[J] += __shfl_xor(rb[J], 32);
-  }
-  reduce_scatter16<NP>(ra, col);
-  // position 16 Ii + mi: part a in group mi >> 2 at the lane whose scatter index is 4 Ii + (mi & 3); part b in
-  // register Ii of lane mi of any group
-  const int ka = 4 * Ii + (mi & 3);
-  const int ca = NP == 16 ? ka : NP == 8 ? 2 * ka : 4 * ka;
-  const float gxa = __shfl(ra[0], 16 * ((mi & 15) >> 2) + (ca & 15));
-  float gxb = 0.f;
-#pragma unroll
-  for (int J = 0; J < nT; ++J) {
-    const float t = __shfl(rb[J], mi & 15);
-    if (Ii == J) gxb = t;
-  }
-  const float hval = valid ? 2.f * (cc0 - (gxa + gxb)) : 0.f;
+  const float hval = valid ? 2.f * (cc0 - gx) : 0.f;
   float mx = fmaxf(gm, fabsf(hval));
   for (int off = 32; off; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
   const float scale = fmaxf(mx, 1e-12f);
@@ -354,46 +318,60 @@ __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restr
       if (o.hNf) o.hNf[(size_t)b * N + p * D + lane] = hs;
     }
   }
-  // 5. G / scale in row-major order, gathered from the tiles in LDS: G_ij sits at tile (i % nT, j % nT), entry
-  // (i / nT, j / nT), or at the mirror entry of tile (j % nT, i % nT). Each lane writes runs of four adjacent entries,
-  // so a wave stores whole rows.
+  // 5. G / scale in row-major order. For each tile row I, lane (grp, col) reads entries (4 grp + k, col), k = 0..3, of
+  // tiles (I, J), J < nT (one 16-byte load each): G[nT (4 grp + k) + I][nT col + J]. It writes rows nT (4 grp + k) + I,
+  // columns nT col .. nT col + nT - 1, so each store instruction covers whole rows.
   if (o.Gi || o.Gf || o.Gh) {
-    const float* Gf32 = reinterpret_cast<const float*>(Gs);
     int32_t* Gib = o.Gi ? o.Gi + (size_t)b * D * D : nullptr;
     int16_t* Ghb = o.Gh ? o.Gh + (size_t)b * D * D : nullptr;
     float* Gfb = o.Gf ? o.Gf + (size_t)b * D * D : nullptr;
-    auto at = [&](int i, int j) -> float {
-      const int ii = i % nT, mi2 = i / nT, jj = j % nT, mj2 = j / nT;
-      const bool up = ii <= jj;
-      const int I = up ? ii : jj, J = up ? jj : ii, mr = up ? mi2 : mj2, mc = up ? mj2 : mi2;
-      return Gf32[((I * nT - I * (I - 1) / 2 + (J - I)) * 64 + mc + 16 * (mr >> 2)) * 4 + (mr & 3)];
-    };
-    if ((D & 3) == 0) {
-      const int runs = D * D / 4;
-      for (int q = lane; q < runs; q += 64) {
-        const int i = q / (D / 4), j0 = 4 * (q % (D / 4));
-        floatx4 gs;
+    const bool full = D == DP;
+    const float invQ = inv * Q;   // rne(G invQ) = rne((G / scale) Q): Q is a power of two
+    // with D = DP, row i = nT (4 grp + k) + I starts at lane base + (nT k + I) DP: the offsets are constants
+    const size_t lb = (size_t)4 * nT * grp * DP + nT * col;
+    int32_t* gil = Gib ? Gib + lb : nullptr;
+    int16_t* ghl = Ghb ? Ghb + lb : nullptr;
+    float* gfl = Gfb ? Gfb + lb : nullptr;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) gs[e] = at(i, j0 + e) * inv;
-        if (Gib || Ghb) {
-          int32_t g4[4];
+    for (int I = 0; I < nT; ++I) {
+      floatx4 blk[nT];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) g4[e] = (int32_t)rintf(gs[e] * Q);
-          if (Gib) *reinterpret_cast<int4*>(Gib + 4 * q) = make_int4(g4[0], g4[1], g4[2], g4[3]);
-          if (Ghb) {
-            const uint32_t lo = (uint32_t)(uint16_t)g4[0] | ((uint32_t)(uint16_t)g4[1] << 16);
-            const uint32_t hi = (uint32_t)(uint16_t)g4[2] | ((uint32_t)(uint16_t)g4[3] << 16);
-            *reinterpret_cast<uint2*>(Ghb + 4 * q) = make_uint2(lo, hi);
+      for (int J = 0; J < nT; ++J)
+        blk[J] = *reinterpret_cast<const floatx4*>(Gt + (I * nT + J) * TS + 16 * col + 4 * grp);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int i = nT * (4 * grp + k) + I, j0 = nT * col;
+        float gs[4];
+        int32_t gq[4];
+#pragma unroll
+        for (int J = 0; J < 4; ++J) {
+          const float g = J < nT ? blk[J % nT][k] : 0.f;
+          gq[J] = (int32_t)rintf(g * invQ);
+          gs[J] = gfl ? g * inv : 0.f;
+        }
+        const int at = (nT * k + I) * DP;
+        if (full && nT == 4) {
+          if (gil) *reinterpret_cast<int4*>(gil + at) = make_int4(gq[0], gq[1], gq[2], gq[3]);
+          if (ghl) {
+            const uint32_t lo = (uint32_t)(uint16_t)gq[0] | ((uint32_t)(uint16_t)gq[1] << 16);
+            const uint32_t hi = (uint32_t)(uint16_t)gq[2] | ((uint32_t)(uint16_t)gq[3] << 16);
+            *reinterpret_cast<uint2*>(ghl + at) = make_uint2(lo, hi);
+          }
+          if (gfl) *reinterpret_cast<floatx4*>(gfl + at) = floatx4{gs[0], gs[1], gs[2], gs[3]};
+        } else if (full && nT == 2) {
+          if (gil) *reinterpret_cast<int2*>(gil + at) = make_int2(gq[0], gq[1]);
+          if (ghl) *reinterpret_cast<uint32_t*>(ghl + at) = (uint32_t)(uint16_t)gq[0] | ((uint32_t)(uint16_t)gq[1] << 16);
+          if (gfl) *reinterpret_cast<float2*>(gfl + at) = make_float2(gs[0], gs[1]);
+        } else if (i < D) {
+#pragma unroll
+          for (int J = 0; J < nT; ++J) {
+            if (j0 + J >= D) continue;
+            const size_t at = (size_t)i * D + j0 + J;
+            if (Gib) Gib[at] = gq[J];
+            if (Ghb) Ghb[at] = (int16_t)gq[J];
+            if (Gfb) Gfb[at] = blk[J][k] * inv;
           }
         }
-        if (Gfb) *reinterpret_cast<floatx4*>(Gfb + 4 * q) = gs;
-      }
-    } else {
-      for (int x = lane; x < D * D; x += 64) {
-        const float gs = at(x / D, x % D) * inv;
-        if (Gib) Gib[x] = (int32_t)rintf(gs * Q);
-        if (Ghb) Ghb[x] = (int16_t)rintf(gs * Q);
-        if (Gfb) Gfb[x] = gs;
       }
     }
   }
@@ -411,8 +389,11 @@ static void launch_mfma(const float* H, const float* y, const float* N0v, float 
 
 hipError_t launch_front_end(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M,
                             int K, float Q, const FrontOut& o, hipStream_t st) {
-  // the expanded Chord inputs and the register-sweep form (form 1) take the register-sweep kernels
-  if (g_front_end_form == 0 && D <= 64 && Mr <= 64 && !o.J && !o.Jf) {
+  // the expanded Chord inputs, the register-sweep form (form 1) and a fixed-point scale Q that is not a power of two
+  // (this kernel rounds G (Q / scale), equal to (G / scale) Q only for Q = 2^k) take the register-sweep kernels
+  int qe = 0;
+  const bool q_pow2 = Q > 0.f && std::frexp(Q, &qe) == 0.5f;
+  if (g_front_end_form == 0 && D <= 64 && Mr <= 64 && !o.J && !o.Jf && q_pow2) {
     if (D <= 16) launch_mfma<16>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);
     else if (D <= 32) launch_mfma<32>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);
     else if (D <= 48) launch_mfma<48>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);

@@ -2,9 +2,11 @@
 """Profiling driver for the detector's kernels, meant to run under rocprofv3.
 
     python3 -m arpeggio_amd.profile_anneal run [--start rand|learned] [--passes 64] [--frames 65536] [--copies 16]
-                                               [--launches 3] [--snr 23.68]
-        draws the 32x32 16-QAM frames once, then runs the full step (front end [+ MLP] + anneal + decode) `launches`
-        times; every kernel of the path appears `launches` times in a trace.
+                                               [--launches 3] [--snr 23.68] [--pipeline plain|adaptive]
+                                               [--stage1 4,24] [--c 6]
+        draws the 32x32 16-QAM frames once, then runs the full step `launches` times; every kernel of the path appears
+        `launches` times in a trace. plain: front end [+ MLP] + anneal + decode. adaptive: the two-stage detector
+        (front end [+ MLP], stage-1 anneal and decode, certificate, stage-2 anneal and decode of the open frames).
     python3 -m arpeggio_amd.profile_anneal copy
         three device copies of 256 MiB: the known-size reference for scaling FETCH_SIZE / WRITE_SIZE.
     python3 -m arpeggio_amd.profile_anneal traffic FETCH_DB WRITE_DB COPY_FETCH_DB COPY_WRITE_DB [--match arpeggio]
@@ -30,7 +32,13 @@ def run(a):
     b = frames.draw(1, a.snr, 0, a.frames, 32, 16, dev='cuda')
     lt = b['l'].to(torch.int8).contiguous()
     key = (0x9E3779B97F4A7C15 + 0xA513) & 0xFFFFFFFFFFFFFFFF
-    pipe = DetectPipeline(a.frames, 64, 64, 16, a.copies, cycles=a.passes, start=a.start, key=key, model=model)
+    if a.pipeline == 'adaptive':
+        from .detector import AdaptivePipeline
+        R1, L1 = (int(x) for x in a.stage1.split(','))
+        pipe = AdaptivePipeline(a.frames, 64, 64, 16, R1=R1, L1=L1, R2=a.copies, L2=a.passes, c=a.c, start=a.start,
+                                key=key, model=model)
+    else:
+        pipe = DetectPipeline(a.frames, 64, 64, 16, a.copies, cycles=a.passes, start=a.start, key=key, model=model)
     for i in range(a.launches):
         pipe.run(b['H'], b['y'], b['N0'], lt)
         torch.cuda.synchronize()
@@ -84,6 +92,9 @@ def main():
     r.add_argument('--copies', type=int, default=16)
     r.add_argument('--launches', type=int, default=3)
     r.add_argument('--snr', type=float, default=23.68)
+    r.add_argument('--pipeline', default='plain', choices=('plain', 'adaptive'))
+    r.add_argument('--stage1', default='4,24', help='adaptive stage 1: anneals,passes')
+    r.add_argument('--c', type=float, default=6.0, help='adaptive certificate constant')
     sub.add_parser('copy')
     t = sub.add_parser('traffic')
     t.add_argument('dbs', nargs=4)

@@ -279,7 +279,7 @@ def roofline_front_end(t_ms, B):
     achieved = flop / (t_ms * 1e-3) / 1e12
     hbm = B * (4 * Mr * D + 4 * Mr + 4 * D * D + 4 * D + D + 4 + 2 * feat_len())
     return dict(bound='mfma', achieved=achieved, peak=MFMA_FP32_TFLOPS, unit='TFLOP/s',
-                frac=achieved / MFMA_FP32_TFLOPS, traffic=None, kernel='front_end_w64_kernel<64>', launch_ms=t_ms,
+                frac=achieved / MFMA_FP32_TFLOPS, traffic=None, kernel='front_end_mfma_kernel<64>', launch_ms=t_ms,
                 algorithmic_flop=flop, hbm_algorithmic_bytes=hbm,
                 hbm=dict(achieved_GBs=hbm / (t_ms * 1e-3) / 1e9, peak_GBs=HBM_PEAK_GBS,
                          frac=hbm / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
