@@ -295,6 +295,24 @@ int arp_mlp_start_fused(const uint16_t *X, int M, int F, const uint16_t *W1t, co
 /* Start thresholds from logits: thr = rint(sigmoid(z * inv_temp) 2^32) clamped to [1, 2^32 - 1]. */
 int arp_start_thresholds(const float *z, long ld, int M, int N, float inv_temp, uint32_t *thr, void *stream);
 
+/* ---- Soft output: max-log bit LLRs, positive favouring bit 1, (B, D nb) dimension-major with the MSB first (nb = 2
+ * for 16-QAM, 1 for QPSK), as arpeggio/mimo/softlib.py:1-9 defines them. */
+
+/* MMSE LLRs from the front end's z and mu (B, D) float32: z = mu l + e with Var(e) = E[l^2] mu (1 - mu); per bit,
+ * (min over levels with the bit 0 - min with it 1) of (z - mu l)^2 / max(Var, 1e-12), halved. Replaces
+ * mimo.mmse_soft_llr (arpeggio/mimo/mimo.py:111-129). */
+int arp_soft_mmse_llr(const float *z, const float *mu, int B, int D, int M, float *llr, void *stream);
+
+/* List LLRs: the candidates of arp_decode_select (R anneal outputs formed from spins as there, then the MMSE levels
+ * xg), the decision (first minimum of ||y - H x||^2, written to lhat (B, D) int8 if not NULL; the C = R + 1 distances
+ * to dist (B, C) if not NULL), and the decision's D nb single-Gray-bit neighbours: LLR = (min d with the bit 0 -
+ * min d with it 1) / N0 clipped to +-clip, N0 per frame (N0, (B)) or N0_scalar. D, Mr <= 128, R <= 63. Replaces
+ * mimo.select + mimo.list_llr (arpeggio/mimo/mimo.py:421-425, 434-462), the list detectors of softlib.detect_llr
+ * (arpeggio/mimo/softlib.py:33-65). */
+int arp_soft_list_llr(const float *H, const float *y, const int8_t *xg, const int8_t *spins, int B, int R, int P, int D,
+                      int Mr, int M, int delta, const float *N0, float N0_scalar, float clip, int8_t *lhat,
+                      float *dist, float *llr, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

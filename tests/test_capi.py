@@ -124,3 +124,18 @@ def test_list_anneal_gram_forms_without_gpu():
     for gt, gh in ((None, None), (p, p)):
         assert L.arp_anneal_arpeggio_list(*args(gt, gh)) == -1 and b'exactly one' in L.arp_last_error()
     assert L.arp_abi_version() == 3
+
+
+def test_soft_argument_errors_without_gpu():
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    assert L.arp_soft_mmse_llr(None, None, 4, 32, 8, None, None) == -1 and b'M must be' in L.arp_last_error()
+    assert L.arp_soft_mmse_llr(None, None, 0, 32, 16, None, None) == 0
+    assert L.arp_soft_list_llr(None, None, None, None, 4, 8, 2, 200, 64, 16, 1, None, 1.0, 50.0, None, None, None,
+                               None) == -3
+    assert L.arp_soft_list_llr(None, None, None, None, 4, 8, 2, 64, 64, 16, 1, None, 1.0, 0.0, None, None, None,
+                               None) == -1 and b'clip' in L.arp_last_error()
+    assert L.arp_soft_list_llr(None, None, None, None, 4, 64, 2, 64, 64, 16, 1, None, 1.0, 50.0, None, None, None,
+                               None) == -3
+    assert L.arp_soft_list_llr(None, None, None, None, 0, 8, 2, 64, 64, 16, 1, None, 1.0, 50.0, None, None, None,
+                               None) == 0
