@@ -271,27 +271,46 @@ def cpu_baseline(b, pipe, args, seg_s=6.0, nseg=3):
                        (chunk, cores, args.passes, args.copies, nseg, seg_s, done, wall, agree))
 
 
-def roofline_front_end(t_ms, B):
+def profile_record(name, key):
+    """A kernel's record from a committed profile summary under profiles/ (None if absent)."""
+    p = os.path.join(ROOT, 'profiles', name)
+    if not os.path.exists(p):
+        return None
+    d = json.load(open(p))
+    return (d.get('launches') or d.get('kernels') or {}).get(key)
+
+
+def roofline_front_end(t_ms, B, learned):
     """The front end: LMMSE + Gram on the matrix cores. Algorithmic work per frame (the Gram, H^T H, and H^T y):
-    2 (2Nr)(2Nt)^2 + 2 (2Nr)(2Nt) FLOP; the bound is the FP32 MFMA peak."""
+    2 (2Nr)(2Nt)^2 + 2 (2Nr)(2Nt) FLOP; the bound is the FP32 MFMA peak. traffic: the measured HBM bytes per launch
+    (FETCH_SIZE + WRITE_SIZE, scaled by a device copy's reading); counters: the matrix-core and vector busy fractions
+    of the same kernel (SQ counters), which also count the inversion's MFMAs."""
     D = Mr = 2 * NT
     flop = B * (2 * Mr * D * D + 2 * Mr * D)
     achieved = flop / (t_ms * 1e-3) / 1e12
-    hbm = B * (4 * Mr * D + 4 * Mr + 4 * D * D + 4 * D + D + 4 + 2 * feat_len())
+    hbm = B * (4 * Mr * D + 4 * Mr + 2 * D * D + 4 * D + D + 4 + (2 * feat_len() if learned else 0))
+    key = ('adaptive' if learned else 'plain') + '/front_end_mfma'
+    tr = profile_record('detect_step_traffic.json', key)
+    ct = profile_record('detect_step_counters.json', key)
     return dict(bound='mfma', achieved=achieved, peak=MFMA_FP32_TFLOPS, unit='TFLOP/s',
-                frac=achieved / MFMA_FP32_TFLOPS, traffic=None, kernel='front_end_mfma_kernel<64>', launch_ms=t_ms,
-                algorithmic_flop=flop, hbm_algorithmic_bytes=hbm,
+                frac=achieved / MFMA_FP32_TFLOPS, traffic=tr['hbm_bytes_per_launch'] if tr else None,
+                kernel='front_end_mfma_kernel<64>', launch_ms=t_ms, algorithmic_flop=flop, hbm_algorithmic_bytes=hbm,
                 hbm=dict(achieved_GBs=hbm / (t_ms * 1e-3) / 1e9, peak_GBs=HBM_PEAK_GBS,
-                         frac=hbm / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
+                         frac=hbm / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
+                counters=dict(mfma_busy=ct['mfma_busy'], vector_busy=ct['vector_busy'],
+                              mfma_per_frame=ct['mfma_per_wave'], source='profiles/detect_step_counters.json')
+                if ct else None,
+                traffic_source='profiles/detect_step_traffic.json' if tr else None)
 
 
 def roofline_anneal(t_ms, B, R, L, flips, learned, kernel):
     """A first-accept anneal launch over B frames, R anneals of L passes each. Algorithmic HBM bytes per launch: G
-    (int32, D x D) + h + inst per frame and T per pass in [+ start thresholds P D int32 per frame]; spins (R P D int8),
-    best energy (int64), (flips, tests) (2 int64) [+ kept flag] per anneal out. The bound that holds is instruction
-    issue; the LDS bytes (one Gram column per flip, the D x D initial-field sweep) are reported beside it."""
+    (int16, D x D, the front end's hand-off) + h + inst per frame and T per pass in [+ start thresholds P D int32 per
+    frame]; spins (R P D int8), best energy (int64), (flips, tests) (2 int64) [+ kept flag] per anneal out. The bound
+    that holds is instruction issue; the LDS bytes (one Gram column per flip, the D x D initial-field sweep) are
+    reported beside it."""
     D, P = 2 * NT, 2
-    hbm = B * (4 * D * D + 4 * D + 4) + 4 * L + B * R * (P * D + 8 + 16)
+    hbm = B * (2 * D * D + 4 * D + 4) + 4 * L + B * R * (P * D + 8 + 16)
     if learned:
         hbm += B * 4 * P * D + B * R
     achieved = hbm / (t_ms * 1e-3) / 1e9
@@ -308,8 +327,10 @@ def roofline_anneal(t_ms, B, R, L, flips, learned, kernel):
                              salu_frac=4 * qs * B * R * L / avail,
                              counters_at=dict(frames=rec['frames'], copies=rec['copies'], passes=rec['passes']),
                              source='profiles/arpeggio_anneal_issue.json')
+    tr = None if learned else profile_record('detect_step_traffic.json', 'plain/arpeggio_fx')
     return dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
-                traffic=None, kernel=kernel, algorithmic_bytes=hbm, launch_ms=t_ms, flips_per_anneal=flips,
+                traffic=tr['hbm_bytes_per_launch'] if tr else None, kernel=kernel, algorithmic_bytes=hbm,
+                launch_ms=t_ms, flips_per_anneal=flips,
                 lds=dict(algorithmic_bytes=lds, achieved_GBs=lds / (t_ms * 1e-3) / 1e9, peak_GBs=LDS_PEAK_GBS,
                          frac=lds / (t_ms * 1e-3) / 1e9 / LDS_PEAK_GBS),
                 issue=issue)
@@ -550,7 +571,7 @@ def main():
         st = out['learned']['stages']
         timed = 'learned (adaptive)'
         if st['front_end'] >= st['anneal_1']:
-            out['roofline'] = roofline_front_end(st['front_end'], B)
+            out['roofline'] = roofline_front_end(st['front_end'], B, True)
         else:
             fl = float(lp.st1.double()[..., 0].mean())
             out['roofline'] = roofline_anneal(st['anneal_1'], B, lp.R1, lp.L1, fl, True,
