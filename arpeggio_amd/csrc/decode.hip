@@ -9,6 +9,7 @@
 #include "kernels.h"
 #include "adaptive.h"
 
+#include <algorithm>
 #include <cstdint>
 
 namespace arp {
@@ -257,6 +258,115 @@ __global__ __launch_bounds__(64) void decode_w64_kernel(const float* __restrict_
   }
 }
 
+// Sum over the wave, the same value in every lane: within each row of 16 lanes by DPP (quad xor 1, quad xor 2, the
+// half-row mirror, the row mirror: each step adds two equal partial sums, so all 16 lanes hold the same row sum), then
+// the four row sums through scalar registers in a fixed order.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_mov<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_mov<0x141>(v);  // row_half_mirror
+  v += dpp_mov<0x140>(v);  // row_mirror
+  const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0));
+  const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16));
+  const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+  const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48));
+  return (r0 + r1) + (r2 + r3);
+}
+
+// The dDI decode of the detector's common shape: two copies, square (Mr = D), D = 32 or 64 as a compile-time
+// constant, H rows 16-byte aligned. H passes through LDS: coalesced 16-byte loads (each instruction reads 1 KiB of
+// consecutive rows), then lane r reads receive row r into registers (row stride D + 4 words: no bank conflicts). Lane
+// j forms coordinate j of every candidate into LDS (over the staged H, once it is in registers); each candidate's
+// residual rows are dot products over the row (no bound tests) and meet in wave_sum. Same selection rule and extras
+// as decode_w64_kernel.
+template <int D>
+__global__ __launch_bounds__(64) void decode_dd_kernel(const float* __restrict__ H, const float* __restrict__ y,
+                                                       const int8_t* __restrict__ xg,
+                                                       const int8_t* __restrict__ spins, int R, int M,
+                                                       const int8_t* __restrict__ ltrue, int8_t* __restrict__ lhat,
+                                                       int32_t* __restrict__ biterr, float* __restrict__ dist_out,
+                                                       DecodeExtra ex) {
+  extern __shared__ float sm[];
+  const int slot = blockIdx.x, lane = threadIdx.x;
+  int b = slot;
+  if (ex.list) {
+    if (slot >= *ex.count) return;
+    b = ex.list[slot];
+  }
+  // read before any lane writes best_d, which may alias prior_d
+  const float pdist = ex.prior_d ? ex.prior_d[b] : 0.f;
+  const int C = R + 1;
+  constexpr int HS = D + 4;   // LDS row stride of the staged H, in words
+  float* Lc = sm;             // C x D candidate levels (after H has left the LDS)
+  const float* Hb = H + (size_t)b * D * D;
+  const bool row = lane < D;
+  constexpr int NV = D * D / 4 / 64;   // 16-byte pieces per lane
+  floatx4 stage[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) stage[k] = reinterpret_cast<const floatx4*>(Hb)[64 * k + lane];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int f = 64 * k + lane, r = f / (D / 4), c4 = f % (D / 4);
+    *reinterpret_cast<floatx4*>(sm + r * HS + 4 * c4) = stage[k];
+  }
+  __syncthreads();
+  float hr[D];
+#pragma unroll
+  for (int q = 0; q < D / 4; ++q) {
+    floatx4 t = {0.f, 0.f, 0.f, 0.f};
+    if (row) t = *reinterpret_cast<const floatx4*>(sm + lane * HS + 4 * q);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) hr[4 * q + e] = t[e];
+  }
+  const float yi = row ? y[(size_t)b * D + lane] : 0.f;
+  const float lmax = M == 4 ? 1.f : 3.f;
+  __syncthreads();   // every lane has its row: the LDS now takes the candidates
+  if (row) {
+    const int x0 = (int)xg[(size_t)b * D + lane];
+    const int8_t* s = spins + (size_t)slot * R * 2 * D + lane;
+    for (int c = 0; c < R; ++c)
+      Lc[c * D + lane] = fminf(fmaxf((float)(x0 + (int)s[(size_t)c * 2 * D] + (int)s[(size_t)c * 2 * D + D]), -lmax),
+                               lmax);
+    Lc[R * D + lane] = (float)x0;
+  }
+  __syncthreads();
+  int best = 0;
+  float bd = 0.f;
+  for (int c = 0; c < C; ++c) {
+    const float* lc = Lc + c * D;
+    float sacc = 0.f;
+#pragma unroll
+    for (int j = 0; j < D; j += 4) {
+      const floatx4 lv = *reinterpret_cast<const floatx4*>(lc + j);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) sacc = __builtin_fmaf(hr[j + e], lv[e], sacc);
+    }
+    const float e = yi - sacc;
+    const float part = wave_sum(row ? e * e : 0.f);
+    if (dist_out && lane == 0) dist_out[(size_t)slot * C + c] = part;
+    if (c == 0 || part < bd) {
+      bd = part;
+      best = c;
+    }
+  }
+  const bool keep = ex.prior_d && !(bd < pdist);   // the prior candidate wins ties
+  if (ex.best_d && lane == 0) ex.best_d[b] = keep ? pdist : bd;
+  int errs = 0;
+  if (row) {
+    const int l = keep ? (int)ex.prior_l[(size_t)b * D + lane] : (int)Lc[best * D + lane];
+    if (lhat) lhat[(size_t)b * D + lane] = (int8_t)l;
+    if (ltrue) errs = __builtin_popcount(gray_bits(l, M) ^ gray_bits(ltrue[(size_t)b * D + lane], M));
+  }
+  if (biterr && ltrue) {
+    for (int o = 32; o; o >>= 1) errs += __shfl_xor(errs, o);
+    if (lane == 0) biterr[b] = errs;
+  }
+}
+
 // Two-wave form for D, Mr <= 128 (up to 64 users): thread i keeps receive row i of H in registers, thread j forms
 // coordinate j of every candidate into LDS, and each candidate's row terms meet in a butterfly per wave and then
 // across the two waves through LDS.
@@ -377,6 +487,17 @@ hipError_t launch_decode_select(const float* H, const float* y, const int8_t* xg
 hipError_t launch_decode_ex(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R, int P,
                             int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat, int32_t* biterr,
                             float* dist_out, const DecodeExtra& ex, hipStream_t st) {
+  const bool aligned = ((reinterpret_cast<uintptr_t>(H) & 15) == 0);
+  if (delta && P == 2 && Mr == D && (D == 64 || D == 32) && aligned) {
+    const size_t lds = sizeof(float) * std::max((size_t)(R + 1) * D, (size_t)D * (D + 4));
+    if (D == 64)
+      hipLaunchKernelGGL((decode_dd_kernel<64>), dim3(B), dim3(64), lds, st, H, y, xg, spins, R, M, ltrue, lhat, biterr,
+                         dist_out, ex);
+    else
+      hipLaunchKernelGGL((decode_dd_kernel<32>), dim3(B), dim3(64), lds, st, H, y, xg, spins, R, M, ltrue, lhat, biterr,
+                         dist_out, ex);
+    return hipGetLastError();
+  }
   if (D <= 64 && Mr <= 64) {
     const int DP = (D + 15) & ~15;
     const size_t l64 = sizeof(float) * (size_t)(R + 1) * DP;

@@ -4,7 +4,7 @@
 //   1. G = H^T H in v_mfma_f32_16x16x4f32 accumulators (the upper 16x16 tiles of the coordinates in tile order,
 //      position 16 I + m = coordinate nT m + I, so that a lane's operand columns are adjacent in memory), H read once
 //      from global memory straight into the operands; c = H^T y by the VALU beside it. A copy of G waits in LDS
-//      (16.3 KiB at D = 64: all 16 tiles, the lower ones as transposes, each column-major) for the outputs.
+//      (10.2 KiB at D = 64: the upper tiles, each column-major) for the outputs.
 //   2. A = G + sigma I (identity on the padding) is swept in place in the accumulators, four pivots at a time. For the
 //      pivot set S = {k0 .. k0+3} the block sweep is
 //          A_SS -> -A_SS^-1,  A_RS -> A_RS A_SS^-1,  A_SR -> A_SS^-1 A_SR,  A_RR -> A_RR - A_RS A_SS^-1 A_SR,
@@ -54,13 +54,14 @@ __device__ constexpr int tile(int I, int J) {
   return I * nT - I * (I - 1) / 2 + (J - I);
 }
 
+// Register budget: three waves per SIMD at D = 64 (161 VGPRs, 12.7 KiB of LDS), four below.
 template <int DP>
-__global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restrict__ H, const float* __restrict__ y,
-                                                            const float* __restrict__ N0v, float N0s, int Mr, int D,
-                                                            int M, int K, float Q, int vec, FrontOut o) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DP >= 64 ? 3 : 4))) void front_end_mfma_kernel(
+    const float* __restrict__ H, const float* __restrict__ y, const float* __restrict__ N0v, float N0s, int Mr, int D,
+    int M, int K, float Q, int vec, FrontOut o) {
   constexpr int nT = DP / 16, nU = nT * (nT + 1) / 2;
   constexpr int TS = 260;                               // tile stride in words: 16 x 16 entries + 4 (bank offset)
-  __shared__ __align__(16) float Gt[nT * nT * TS];      // the full Gram, tile by tile, each tile column-major
+  __shared__ __align__(16) float Gt[nU * TS];           // the Gram's upper tiles, each column-major
   __shared__ __align__(16) float pan[2][DP * 4];        // panel rows S: pan[p * 4 + t] = A[k0 + t][p]
   __shared__ __align__(16) float bc[64];                // x_g in tile order
   const int b = blockIdx.x, lane = threadIdx.x, col = lane & 15, grp = lane >> 4;
@@ -125,20 +126,11 @@ __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restr
     c[I] = v;
     c0[I] = v;
   }
-  // G waits in LDS for the outputs as the full tile set, the lower tiles the transposes of the upper ones: tile (I, J)
-  // entry (r, c) = G[nT r + I][nT c + J] at Gt[(I nT + J) TS + 16 c + r]. Lane (grp, col) holds entries (4 grp + v,
-  // col) of upper tile (I, J): one 16-byte store, and four single stores of the mirror entries of tile (J, I).
+  // G waits in LDS for the outputs as its upper tiles: tile (I, J), I <= J, entry (r, c) = G[nT r + I][nT c + J] at
+  // Gt[tile(I, J) TS + 16 c + r]. Lane (grp, col) holds entries (4 grp + v, col): one 16-byte store per tile. 12.7 KiB
+  // of LDS per wave in all, so that three waves fit a SIMD.
 #pragma unroll
-  for (int I = 0; I < nT; ++I)
-#pragma unroll
-    for (int J = I; J < nT; ++J) {
-      const floatx4 t = T[tile<nT>(I, J)];
-      *reinterpret_cast<floatx4*>(Gt + (I * nT + J) * TS + 16 * col + 4 * grp) = t;
-      if (I < J) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Gt[(J * nT + I) * TS + 16 * (4 * grp + v) + col] = t[v];
-      }
-    }
+  for (int u = 0; u < nU; ++u) *reinterpret_cast<floatx4*>(Gt + u * TS + 16 * col + 4 * grp) = T[u];
   const float el2 = M == 4 ? 1.f : 5.f;
   const float n0 = N0v ? N0v[b] : N0s;
   const float sigma = (n0 / 2.f) / el2;
@@ -253,7 +245,7 @@ __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restr
     }
   }
   float gii = 0.f;
-  if (valid) gii = Gt[(Ii * nT + Ii) * TS + 17 * mi];
+  if (valid) gii = Gt[(Ii * nT - Ii * (Ii - 1) / 2) * TS + 17 * mi];
   const float mu = 1.f - sigma * dgi;
   const float vq = z / mu;
   const float lmax = M == 4 ? 1.f : 3.f;
@@ -280,26 +272,30 @@ __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restr
   }
 
   // 4. h = 2 H^T (y - H x_g) (mimo.ising_delta_at) as 2 (c - G x_g): lane i = nT m + I takes the dot product of row
-  // i with x_g, reading the row from tiles (I, J), entries (m, c) at 16 c + m (the lanes of one I read 16 consecutive
-  // words and the nT values of I sit 16 banks apart), and x_g in tile order as broadcasts; the row's largest |G|
-  // rides along. scale = max(max|h|, max|G|), clamped at 1e-12.
+  // i with x_g (x_g in tile order, read as broadcasts). Row i is row m of tiles (I, J >= I), entries (m, c) at 16 c + m,
+  // and column m of tiles (J < I, I), entries (c, m) at 16 m + c: a per-lane base and stride for each J. The row's
+  // largest |G| rides along. scale = max(max|h|, max|G|), clamped at 1e-12.
   if (lane < DP) bc[16 * Ii + mi] = l;
   __syncthreads();
   float gx = 0.f, gm = 0.f;
   if (lane < DP) {
-    const float* grow = Gt + Ii * nT * TS + mi;
 #pragma unroll
-    for (int J = 0; J < nT; ++J)
+    for (int J = 0; J < nT; ++J) {
+      const bool up = J >= Ii;
+      const int tl = up ? Ii * nT - Ii * (Ii - 1) / 2 + (J - Ii) : J * nT - J * (J - 1) / 2 + (Ii - J);
+      const float* g0 = Gt + tl * TS + (up ? mi : 16 * mi);
+      const int st = up ? 16 : 1;
 #pragma unroll
       for (int c4 = 0; c4 < 16; c4 += 4) {
         const floatx4 xv = *reinterpret_cast<const floatx4*>(bc + 16 * J + c4);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const float g = grow[J * TS + 16 * (c4 + e)];
+          const float g = g0[st * (c4 + e)];
           gx = __builtin_fmaf(g, xv[e], gx);
           gm = fmaxf(gm, fabsf(g));   // padding entries are zero
         }
       }
+    }
   }
   const float hval = valid ? 2.f * (cc0 - gx) : 0.f;
   float mx = fmaxf(gm, fabsf(hval));
@@ -318,9 +314,10 @@ __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restr
       if (o.hNf) o.hNf[(size_t)b * N + p * D + lane] = hs;
     }
   }
-  // 5. G / scale in row-major order. For each tile row I, lane (grp, col) reads entries (4 grp + k, col), k = 0..3, of
-  // tiles (I, J), J < nT (one 16-byte load each): G[nT (4 grp + k) + I][nT col + J]. It writes rows nT (4 grp + k) + I,
-  // columns nT col .. nT col + nT - 1, so each store instruction covers whole rows.
+  // 5. G / scale in row-major order. For each tile row I, lane (grp, col) reads G[nT (4 grp + k) + I][nT col + J],
+  // k = 0..3, J < nT: entries (4 grp + k, col) of tile (I, J) for J >= I (one 16-byte load), entries (col, 4 grp + k)
+  // of tile (J, I) for J < I (four loads at a constant stride). It writes rows nT (4 grp + k) + I, columns
+  // nT col .. nT col + nT - 1, so each store instruction covers whole rows.
   if (o.Gi || o.Gf || o.Gh) {
     int32_t* Gib = o.Gi ? o.Gi + (size_t)b * D * D : nullptr;
     int16_t* Ghb = o.Gh ? o.Gh + (size_t)b * D * D : nullptr;
@@ -336,8 +333,14 @@ __global__ __launch_bounds__(64) void front_end_mfma_kernel(const float* __restr
     for (int I = 0; I < nT; ++I) {
       floatx4 blk[nT];
 #pragma unroll
-      for (int J = 0; J < nT; ++J)
-        blk[J] = *reinterpret_cast<const floatx4*>(Gt + (I * nT + J) * TS + 16 * col + 4 * grp);
+      for (int J = 0; J < nT; ++J) {
+        if (J >= I) {
+          blk[J] = *reinterpret_cast<const floatx4*>(Gt + tile<nT>(I, J) * TS + 16 * col + 4 * grp);
+        } else {
+          const float* g0 = Gt + tile<nT>(J, I) * TS + 64 * grp + col;
+          blk[J] = floatx4{g0[0], g0[16], g0[32], g0[48]};
+        }
+      }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int i = nT * (4 * grp + k) + I, j0 = nT * col;
