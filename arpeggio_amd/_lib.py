@@ -68,6 +68,7 @@ SIGNATURES = {
                             _u32, _P, _i32, _P, _P, _P, _P, ctypes.c_size_t, _P],
     'arp_detect_adaptive_noalloc': [_P, _P, _P, _f, _i32, _i32, _i32, _i32, _P, _i32, _i32, _P, _i32, _i32, _i32, _f,
                                     _P, _u32, _u32, _P, _i32, _P, _P, _P, _P],
+    'arp_ising_ri': [_P, _P, _P, _i32, _i32, _i32, _i32, _f, _f, _P, _P, _P, _P, _P],
     'arp_soft_mmse_llr': [_P, _P, _i32, _i32, _i32, _P, _P],
     'arp_soft_list_llr': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _P, _f, _f, _P, _P, _P, _P],
 }

@@ -295,6 +295,15 @@ int arp_mlp_start_fused(const uint16_t *X, int M, int F, const uint16_t *W1t, co
 /* Start thresholds from logits: thr = rint(sigmoid(z * inv_temp) 2^32) clamped to [1, 2^32 - 1]. */
 int arp_start_thresholds(const float *z, long ld, int M, int N, float inv_temp, uint32_t *thr, void *stream);
 
+/* RI-MIMO Ising build in fixed point: mimo.ising (arpeggio/mimo/mimo.py:132-149) and chordfx.to_fixed's J, h, h0
+ * (arpeggio/mimo/chordfx.py:60-76), the problem fxdetect.detect anneals for its 'ri' schedules (fxdetect.py:17-18,
+ * 28-30). H (B, Mr, D), y (B, Mr), z (B, D) the LMMSE output (float32). Spins: N = D (QPSK) or N = 2D (16-QAM, MSB
+ * block first; l = 2 s_MSB + s_LSB). Out: J (B, N, N) int16 = rne(-offdiag(T^T H^T H T) / scale Q), h (B, N) int32 =
+ * rne((h0 + 2 r s_P) Q), h0 (B, N) int32 = rne(2 T^T H^T y / scale Q), scale (B) float32 or NULL; s_P are the spins of
+ * z's nearest levels (ties to the lower). D, Mr <= 128, N <= 256. */
+int arp_ising_ri(const float *H, const float *y, const float *z, int B, int Mr, int D, int M, float r, float Q,
+                 int16_t *J, int32_t *h, int32_t *h0, float *scale, void *stream);
+
 /* ---- Soft output: max-log bit LLRs, positive favouring bit 1, (B, D nb) dimension-major with the MSB first (nb = 2
  * for 16-QAM, 1 for QPSK), as arpeggio/mimo/softlib.py:1-9 defines them. */
 

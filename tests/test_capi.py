@@ -139,3 +139,13 @@ def test_soft_argument_errors_without_gpu():
                                None) == -3
     assert L.arp_soft_list_llr(None, None, None, None, 0, 8, 2, 64, 64, 16, 1, None, 1.0, 50.0, None, None, None,
                                None) == 0
+
+
+def test_ising_ri_argument_errors_without_gpu():
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    assert L.arp_ising_ri(None, None, None, 4, 64, 64, 8, 0.15, 4096.0, None, None, None, None, None) == -1
+    assert L.arp_ising_ri(None, None, None, 4, 64, 200, 16, 0.15, 4096.0, None, None, None, None, None) == -3
+    assert L.arp_ising_ri(None, None, None, 4, 64, 64, 16, 0.15, 4096.0, None, None, None, None, None) == -1
+    assert b'null' in L.arp_last_error()
+    assert L.arp_ising_ri(None, None, None, 0, 64, 64, 16, 0.15, 4096.0, None, None, None, None, None) == 0
