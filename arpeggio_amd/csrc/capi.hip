@@ -361,7 +361,8 @@ extern "C" int arp_anneal_arpeggio_list(const int32_t* Gt, const int16_t* Gh, co
   if (B == 0) return ok();
   if (!Gt == !Gh) return fail(ARP_E_ARG, "arpeggio_list: exactly one of Gt (int32) and Gh (int16) must be given");
   if (!h || !T || !inst || !out_s || !out_de || !out_st) return fail(ARP_E_ARG, "arpeggio_list: null pointer");
-  const int waves = arpeggio_waves(R, D);
+  // a frame list is usually short (the open frames of a two-stage detection): more anneals in flight per frame
+  const int waves = list && g_waves_override == 0 ? (R < 8 ? R : 8) : arpeggio_waves(R, D);
   if (!block_fits(waves, ARP_ARPEGGIO_MAX_THREADS))
     return fail(ARP_E_ARG, "arpeggio_list: %d anneals per workgroup (%d threads) exceed the kernel's bound of %d "
                 "threads", waves, ARP_WAVE * waves, ARP_ARPEGGIO_MAX_THREADS);

@@ -247,3 +247,6 @@ def train(model, feats, labels, epochs=4, batch=8192, lr=2e-3, seed=0, log=None)
 
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'weights', 'warmstart_32x32_16qam.npz')
+# 64 users x 64 rx, 16-QAM, Kronecker rho = 0.5 at both ends, labels and schedule T 0.08 -> 0.03 (train_warmstart
+# --users 64 --rho 0.5 --T0 0.08 --T1 0.03 --snr-lo 24 --snr-hi 32)
+WEIGHTS_64 = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'weights', 'warmstart_64x64_16qam_rho05.npz')

@@ -465,32 +465,66 @@ def batched(args):
                 ber=det.bit_errors() / nbits(fr), R=8, passes=64, snr_db=[8.0, 24.0], graph=True)
 
 
+SNR_64 = 28.0                                       # where the annealers beat LMMSE on this channel
+SCHED_64 = dict(T0=0.08, T1=0.03, q=0.0)            # first accept, tuned at 26-28 dB on 4,096 frames
+CHORD_64 = dict(T0=0.2, T1=0.2, q=0.4, kappa=0.3)   # Chord re-tuned on the same frames (256 steps)
+
+
 def users64(args):
-    """64 users x 64 rx, 16-QAM, Kronecker exponential correlation rho = 0.5 at both ends (GPU generator), 20 dB,
-    2^16 frames, 32 anneals per frame: plain Arpeggio at 64 and 128 passes, K timed steps each after warm-up; and
-    Chord at 256 steps (t 0.3 -> 0.2, q 0.4, kappa 0.3) on the first 4,096 of the frames, 3 timed steps."""
+    """64 users x 64 rx, 16-QAM, Kronecker exponential correlation rho = 0.5 at both ends (GPU generator), 28 dB (where
+    the annealers beat the LMMSE decision), 2^16 frames, 32 anneals per frame, first accept with T 0.08 -> 0.03: plain
+    (random start) at 64 and 128 passes; the learned start (the 64-user network) at 32 and 64 passes, each with its
+    paired BER ratio to plain at 128 passes, and the speedup of the smallest learned budget whose upper 95 % bound is
+    <= 1.10; Chord re-tuned (256 steps, t 0.2, q 0.4, kappa 0.3) on 4,096 of the frames; the LMMSE decision's BER."""
     import torch
     from arpeggio_amd import frames
-    from arpeggio_amd.detector import DetectPipeline
+    from arpeggio_amd.detector import DetectPipeline, front_end
+    from arpeggio_amd.learned import WEIGHTS_64, WarmStartMLP
+    from arpeggio_amd.stats import boot_ratio
     B, nt = args.frames_64u, 64
-    b = frames.draw_device(args.seed + 64, 20.0, 20.0, B, nt, 16, rho_rx=0.5, rho_tx=0.5)
-    out = {}
-    for name, kw in (('arpeggio_64', dict(policy='arpeggio', cycles=64, T0=0.1, T1=0.1, q=0.0)),
-                     ('arpeggio_128', dict(policy='arpeggio', cycles=128, T0=0.1, T1=0.1, q=0.0))):
-        pipe = DetectPipeline(B, 2 * nt, 2 * nt, 16, 32, key=key_of(args.seed), inst0=0, **kw)
+    b = frames.draw_device(args.seed + 64, SNR_64, SNR_64, B, nt, 16, rho_rx=0.5, rho_tx=0.5)
+    lt = b['l'].to(torch.int8).contiguous()
+    nbits = B * 4 * nt
+    o = front_end(b['H'], b['y'], b['N0'], 16, 1, want=('xg',))
+    gb = lambda v: torch.stack([v > 0, v.abs() == 1], -1)
+    lmmse = float((gb(o['xg'].float()) != gb(b['l'])).sum()) / nbits
+    del o
+    model = WarmStartMLP.load(WEIGHTS_64)
+    out, errs = {}, {}
+    for name, start, L in (('plain_64', 'rand', 64), ('plain_128', 'rand', 128), ('learned_32', 'learned', 32),
+                           ('learned_64', 'learned', 64)):
+        pipe = DetectPipeline(B, 2 * nt, 2 * nt, 16, 32, key=key_of(args.seed), inst0=0, policy='arpeggio', cycles=L,
+                              start=start, model=model if start == 'learned' else None, **SCHED_64)
+        _, be = pipe.run(b['H'], b['y'], b['N0'], lt)
+        errs[name] = be.cpu().numpy().astype(np.int64)
         t = time_pipe(pipe, b, args.steps, args.warmup, args.warm_seconds)
-        out[name] = dict(value=t['value'], ms_per_step=t['ms_per_step'], anneal_ms=t['anneal_ms'], ber=t['ber'])
+        out[name] = dict(value=t['value'], ms_per_step=t['ms_per_step'], anneal_ms=t['anneal_ms'],
+                         ber=float(errs[name].sum()) / nbits, passes=L, start=start)
         del pipe
+    ref = errs['plain_128']
+    chosen = None
+    for name in ('learned_32', 'learned_64'):
+        r, lo, hi = boot_ratio(errs[name], ref)
+        out[name].update(ratio_to_plain_128=r, ci=[lo, hi], ber_matched=bool(hi <= EQUIV_LIMIT))
+        if chosen is None and hi <= EQUIV_LIMIT:
+            chosen = name
+    log('64 users at %.0f dB: LMMSE %.3e, plain 64/128 passes %.3e / %.3e, learned 32/64 %.3e / %.3e' % (
+        SNR_64, lmmse, out['plain_64']['ber'], out['plain_128']['ber'], out['learned_32']['ber'],
+        out['learned_64']['ber']))
     Bc = min(B, 4096)
     bc = {k: (v[:Bc] if torch.is_tensor(v) and v.dim() > 0 and v.shape[0] == B else v) for k, v in b.items()}
     pipe = DetectPipeline(Bc, 2 * nt, 2 * nt, 16, 32, key=key_of(args.seed), inst0=0, policy='chord', cycles=256,
-                          T0=0.3, T1=0.2, q=0.4, kappa=0.3)
+                          **CHORD_64)
     t = time_pipe(pipe, bc, 3, 1, 0.0)
     out['chord_256'] = dict(value=t['value'], ms_per_step=t['ms_per_step'], anneal_ms=t['anneal_ms'], ber=t['ber'],
-                            frames=Bc)
+                            frames=Bc, **CHORD_64)
     del pipe
-    return dict(unit='detections/s', frames=B, users=nt, rx=nt, M=16, copies=32, snr_db=20.0, rho=0.5,
-                data='GPU generator, Kronecker-correlated i.i.d. Rayleigh', **out)
+    res = dict(unit='detections/s', frames=B, users=nt, rx=nt, M=16, copies=32, snr_db=SNR_64, rho=0.5,
+               schedule=SCHED_64, lmmse_ber=lmmse, data='GPU generator, Kronecker-correlated Rayleigh', **out)
+    if chosen:
+        res['learned_fixed_ber'] = dict(budget=chosen, value=out[chosen]['value'],
+                                        speedup_vs_plain_128=out[chosen]['value'] / out['plain_128']['value'])
+    return res
 
 
 def main():

@@ -101,7 +101,8 @@ def test_adam_and_thresholds():
     assert np.abs(got - want).max() <= 1
 
 
-@pytest.mark.parametrize('D,P,gate', [(64, 2, True), (64, 2, False), (32, 4, True), (100, 2, True), (17, 6, True)])
+@pytest.mark.parametrize('D,P,gate', [(64, 2, True), (64, 2, False), (32, 4, True), (100, 2, True), (128, 2, True),
+                                      (17, 6, True)])
 def test_arpeggio_ws_vs_oracle(orc, D, P, gate):
     from arpeggio_amd import _lib
     from test_gpu_anneal import random_problem
@@ -194,3 +195,32 @@ def test_fused_inference_equals_gemm_path(M):
     b = model.thresholds(X, 0.35, fused=False)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+def test_trained_start_gains_at_64_users():
+    """The 64-user network (Kronecker rho = 0.5, D = 128, unfused GEMM inference): on held-out frames at 28 dB the
+    learned start at 64 passes beats the random start at 128 passes (fewer than half the passes), and at 32 passes its
+    BER is less than half the random start's."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.detector import DetectPipeline
+    from arpeggio_amd.learned import WEIGHTS_64, WarmStartMLP
+    from arpeggio_amd.stats import boot_ratio
+    model = WarmStartMLP.load(WEIGHTS_64)
+    assert model.D == 128
+    B = 8192
+    b = frames.draw_device(7042, 28.0, 28.0, B, 64, 16, rho_rx=0.5, rho_tx=0.5)
+    lt = b['l'].to(torch.int8).contiguous()
+    be = {}
+    for start, L in (('rand', 32), ('rand', 128), ('learned', 32), ('learned', 64)):
+        pipe = DetectPipeline(B, 128, 128, 16, 32, cycles=L, T0=0.08, T1=0.03, start=start, key=41,
+                              model=model if start == 'learned' else None)
+        _, e = pipe.run(b['H'], b['y'], b['N0'], lt)
+        torch.cuda.synchronize()
+        be[(start, L)] = e.cpu().numpy().astype(np.int64)
+        del pipe
+    r32, _, hi32 = boot_ratio(be[('learned', 32)], be[('rand', 32)])
+    r64, _, hi64 = boot_ratio(be[('learned', 64)], be[('rand', 128)])
+    say('64 users: learned/rand at 32 passes %.3f (hi %.3f); learned 64 / rand 128 %.3f (hi %.3f)' % (r32, hi32, r64,
+                                                                                                     hi64))
+    assert hi32 < 0.5
+    assert hi64 < 1.0
