@@ -55,16 +55,19 @@ __device__ constexpr int tile(int I, int J) {
 }
 
 // Register budget: three waves per SIMD at D = 64 (161 VGPRs, 12.7 KiB of LDS), four below.
-template <int DP>
+// GIVEN: the MDI sub-problem build around a given decision (mimo.mdi's dDI step, arpeggio/mimo/mimo.py:570-590):
+// x_g is read from xg_in instead of computed, the inversion (phases 2-3) is skipped, and workgroup i builds frame
+// list[i] (written at the frame's index).
+template <int DP, bool GIVEN>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DP >= 64 ? 3 : 4))) void front_end_mfma_kernel(
     const float* __restrict__ H, const float* __restrict__ y, const float* __restrict__ N0v, float N0s, int Mr, int D,
-    int M, int K, float Q, int vec, FrontOut o) {
+    int M, int K, float Q, int vec, FrontOut o, const int8_t* __restrict__ xg_in, const int32_t* __restrict__ list) {
   constexpr int nT = DP / 16, nU = nT * (nT + 1) / 2;
   constexpr int TS = 260;                               // tile stride in words: 16 x 16 entries + 4 (bank offset)
   __shared__ __align__(16) float Gt[nU * TS];           // the Gram's upper tiles, each column-major
   __shared__ __align__(16) float pan[2][DP * 4];        // panel rows S: pan[p * 4 + t] = A[k0 + t][p]
   __shared__ __align__(16) float bc[64];                // x_g in tile order
-  const int b = blockIdx.x, lane = threadIdx.x, col = lane & 15, grp = lane >> 4;
+  const int b = GIVEN ? list[blockIdx.x] : (int)blockIdx.x, lane = threadIdx.x, col = lane & 15, grp = lane >> 4;
   const float* Hb = H + (size_t)b * Mr * D;
   const float* yb = y + (size_t)b * Mr;
 
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DP >= 64 ? 3
   // sweep to themselves); the panel buffer alternates between two halves, so a block's stores never meet the
   // previous block's loads.
 #pragma unroll
-  for (int blk = 0; blk < DP / 4; ++blk) {
+  for (int blk = 0; blk < (GIVEN ? 0 : DP / 4); ++blk) {
     const int k0 = 4 * blk, J0 = k0 / 16, g0 = (k0 % 16) / 4;
     float* P = pan[blk & 1];
     // rows k0 + s of tiles (J0, I >= J0) sit in group g0, register s; for I < J0 the same entries A[j][k0 + s] are
@@ -251,13 +254,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DP >= 64 ? 3
   const float lmax = M == 4 ? 1.f : 3.f;
   float l = fminf(fmaxf(2.f * ceilf(vq * 0.5f) - 1.f, -lmax), lmax);
   if (vq != vq) l = -lmax;
+  if (GIVEN) l = valid ? (float)xg_in[(size_t)b * D + lane] : 0.f;
   if (!valid) l = 0.f;
-  if (valid) {
+  if (valid && !GIVEN) {
     if (o.z) o.z[(size_t)b * D + lane] = z;
     if (o.mu) o.mu[(size_t)b * D + lane] = mu;
     if (o.xg) o.xg[(size_t)b * D + lane] = (int8_t)l;
   }
-  if (o.feat) {
+  if (o.feat && !GIVEN) {
     float gs = gii;
     for (int off = 32; off; off >>= 1) gs += __shfl_xor(gs, off);
     const float gbar = gs / (float)D;
@@ -380,14 +384,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DP >= 64 ? 3
   }
 }
 
-template <int DP>
+template <int DP, bool GIVEN = false>
 static void launch_mfma(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M,
-                        int K, float Q, const FrontOut& o, hipStream_t st) {
+                        int K, float Q, const FrontOut& o, hipStream_t st, const int8_t* xg_in = nullptr,
+                        const int32_t* list = nullptr) {
   constexpr int nT = DP / 16;
   // vector loads of a lane's nT adjacent columns: whole rows of nT-aligned width and an aligned base
   const size_t align = nT == 4 ? 16 : nT == 2 ? 8 : 4;
   const int vec = (D == DP) && ((reinterpret_cast<uintptr_t>(H) % align) == 0);
-  hipLaunchKernelGGL((front_end_mfma_kernel<DP>), dim3(B), dim3(64), 0, st, H, y, N0v, N0s, Mr, D, M, K, Q, vec, o);
+  hipLaunchKernelGGL((front_end_mfma_kernel<DP, GIVEN>), dim3(B), dim3(64), 0, st, H, y, N0v, N0s, Mr, D, M, K, Q, vec,
+                     o, xg_in, list);
+}
+
+bool mdi_build_mfma(const float* H, const float* y, const int8_t* xg, const int32_t* list, int n, int Mr, int D,
+                    float Q, int32_t* Gi, int32_t* hi, hipStream_t st) {
+  int qe = 0;
+  if (D > 64 || Mr > 64 || !(Q > 0.f) || std::frexp(Q, &qe) != 0.5f) return false;
+  FrontOut o{};
+  o.Gi = Gi;
+  o.hi = hi;
+  if (D <= 16) launch_mfma<16, true>(H, y, nullptr, 1.f, n, Mr, D, 16, 1, Q, o, st, xg, list);
+  else if (D <= 32) launch_mfma<32, true>(H, y, nullptr, 1.f, n, Mr, D, 16, 1, Q, o, st, xg, list);
+  else if (D <= 48) launch_mfma<48, true>(H, y, nullptr, 1.f, n, Mr, D, 16, 1, Q, o, st, xg, list);
+  else launch_mfma<64, true>(H, y, nullptr, 1.f, n, Mr, D, 16, 1, Q, o, st, xg, list);
+  return true;
 }
 
 hipError_t launch_front_end(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M,

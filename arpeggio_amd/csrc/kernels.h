@@ -63,6 +63,10 @@ hipError_t launch_front_end(const float* H, const float* y, const float* N0v, fl
 hipError_t launch_front_end_sweep(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D,
                                   int M, int K, float Q, const FrontOut& o, hipStream_t st);
 void set_front_end_form(int form);
+// The MDI sub-problem build (Gi, hi around a given decision xg for the frames of a list) on the matrix-core front end;
+// false (nothing launched) for D or Mr > 64 or a Q that is not a power of two.
+bool mdi_build_mfma(const float* H, const float* y, const int8_t* xg, const int32_t* list, int n, int Mr, int D,
+                    float Q, int32_t* Gi, int32_t* hi, hipStream_t st);
 hipError_t launch_decode_select(const float* H, const float* y, const int8_t* xg, const int8_t* spins, int B, int R,
                                 int P, int D, int Mr, int M, int delta, const int8_t* ltrue, int8_t* lhat,
                                 int32_t* biterr, float* dist_out, hipStream_t st);

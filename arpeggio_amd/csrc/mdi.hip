@@ -15,6 +15,7 @@
 // the two streams of a user start with equal error variances, so which of them goes first is decided by rounding, in the
 // reference as here.
 #include "mdi.h"
+#include "kernels.h"
 
 namespace arp {
 
@@ -264,7 +265,7 @@ hipError_t launch_mmse_sic(const float* H, const float* y, const float* N0v, flo
                            int per_user, int8_t* xs, hipStream_t st) {
   const size_t lds = sizeof(float) * ((size_t)Mr * D + (size_t)D * (D + 1) + 2 * D + 4) + sizeof(int) * D;
   if (lds > 65536)
-    hipFuncSetAttribute((const void*)mmse_sic_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)mmse_sic_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(mmse_sic_kernel, dim3(B), dim3(threads_for(D)), lds, st, H, y, N0v, N0s, Mr, D, M, per_user, xs);
   return hipGetLastError();
 }
@@ -277,9 +278,11 @@ hipError_t launch_mdi_radius(const float* H, const float* y, const int8_t* xg, i
 
 hipError_t launch_mdi_build(const float* H, const float* y, const int8_t* xg, const int32_t* list, int n, int Mr, int D,
                             float Q, int32_t* Gi, int32_t* hi, hipStream_t st) {
+  // up to 32 users: the matrix-core front end in its given-decision form (Gram on MFMA, no inversion)
+  if (mdi_build_mfma(H, y, xg, list, n, Mr, D, Q, Gi, hi, st)) return hipGetLastError();
   const size_t lds = sizeof(float) * ((size_t)Mr * D + Mr + D + 2);
   if (lds > 65536)
-    hipFuncSetAttribute((const void*)mdi_build_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)mdi_build_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(mdi_build_kernel, dim3(n), dim3(threads_for(D)), lds, st, H, y, xg, list, n, Mr, D, Q, Gi, hi);
   return hipGetLastError();
 }
