@@ -28,6 +28,13 @@ class BatchClass(ctypes.Structure):
                 ('start', ctypes.c_int), ('tile', ctypes.c_int)]
 
 
+class BatchFrames(ctypes.Structure):
+    """arp_batch_frames: one class of a mixed batch for the whole-chain entry point."""
+    _fields_ = [('H', ctypes.c_void_p), ('y', ctypes.c_void_p), ('N0', ctypes.c_void_p), ('N0_scalar', ctypes.c_float),
+                ('inst', ctypes.c_void_p), ('ltrue', ctypes.c_void_p), ('lhat', ctypes.c_void_p),
+                ('biterr', ctypes.c_void_p), ('nt', ctypes.c_int), ('M', ctypes.c_int), ('B', ctypes.c_int)]
+
+
 SIGNATURES = {
     'arp_philox_kat': [_P, _P, _i32, _P],
     'arp_set_anneal_waves': [_i32],
@@ -69,12 +76,14 @@ SIGNATURES = {
     'arp_detect_adaptive_noalloc': [_P, _P, _P, _f, _i32, _i32, _i32, _i32, _P, _i32, _i32, _P, _i32, _i32, _i32, _f,
                                     _P, _u32, _u32, _P, _i32, _P, _P, _P, _P],
     'arp_ising_ri': [_P, _P, _P, _i32, _i32, _i32, _i32, _f, _f, _P, _P, _P, _P, _P],
+    'arp_detect_batched': [_P, _i32, _P, _i32, _i32, _i32, _f, _u32, _u32, _i32, _i32, _i32, _P, ctypes.c_size_t, _P],
+    'arp_tune_batched': [_P, _i32, _P, _i32, _i32, _i32, _f, _u32, _u32, _i32, _P, ctypes.c_size_t, _P, _P, _P],
     'arp_soft_mmse_llr': [_P, _P, _i32, _i32, _i32, _P, _P],
     'arp_soft_list_llr': [_P, _P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _P, _f, _f, _P, _P, _P, _P],
 }
 
 # entry points whose return type is not int (bound in load())
-SPECIAL = {'arp_detect_workspace_bytes'}
+SPECIAL = {'arp_detect_workspace_bytes', 'arp_detect_batched_workspace_bytes'}
 
 _lib = None
 
@@ -97,6 +106,8 @@ def load(path=None):
     L.arp_abi_version.restype = ctypes.c_int
     L.arp_detect_workspace_bytes.restype = ctypes.c_size_t
     L.arp_detect_workspace_bytes.argtypes = [_i32, _i32, _i32, _i32, _i32]
+    L.arp_detect_batched_workspace_bytes.restype = ctypes.c_size_t
+    L.arp_detect_batched_workspace_bytes.argtypes = [_P, _i32, _i32]
     for name, args in SIGNATURES.items():
         f = getattr(L, name)
         f.argtypes = args

@@ -137,6 +137,52 @@ class BatchedDetector:
                                            self.P, c['D'], c['Mr'], c['M'], 1, P_(lt), P_(c['lhat']),
                                            P_(c['biterr']) if lt is not None else None, None, s), 'decode_select')
 
+    def _c_classes(self, frames, with_errors):
+        """The arp_batch_frames table of the whole-chain C entry point (decisions into c['lhat_c'], errors into
+        c['biterr_c'])."""
+        arr = (_lib.BatchFrames * len(self.cls))()
+        keep = []
+        for i, (c, f) in enumerate(zip(self.cls, frames)):
+            n0t, n0s = _n0_arg(f['N0'], c['B'])
+            keep.append(n0t)
+            if 'lhat_c' not in c:
+                c['lhat_c'] = torch.empty_like(c['lhat'])
+                c['biterr_c'] = torch.empty_like(c['biterr'])
+            lt = f.get('lt') if with_errors else None
+            p = lambda t: t.data_ptr() if t is not None else None
+            arr[i] = _lib.BatchFrames(p(f['H']), p(f['y']), p(n0t), n0s if n0t is None else 0.0, p(c['inst']), p(lt),
+                                      p(c['lhat_c']), p(c['biterr_c']) if lt is not None else None, c['nt'], c['M'],
+                                      c['B'])
+        return arr, keep
+
+    def run_c(self, frames, with_errors=True, stream=None):
+        """The same step through the single C entry point arp_detect_batched (its own workspace, this detector's launch
+        shape); decisions and bit errors land in each class's 'lhat_c' and 'biterr_c'."""
+        arr, keep = self._c_classes(frames, with_errors)
+        need = self.lib.arp_detect_batched_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(self.cls), self.R)
+        if getattr(self, '_work', None) is None or self._work.numel() < need:
+            self._work = torch.empty(need, dtype=torch.uint8, device='cuda')
+        wg, waves, budget = self.shape
+        _lib.check(self.lib.arp_detect_batched(ctypes.cast(arr, ctypes.c_void_p), len(self.cls), _lib.ptr(self.T),
+                                               self.L, self.q, self.R, self.Q, self.key & MASK, (self.key >> 32) & MASK,
+                                               wg, waves, budget, _lib.ptr(self._work), self._work.numel(),
+                                               _lib.stream_ptr(stream)), 'detect_batched')
+        self._keep = (arr, keep)
+
+    def tune_c(self, frames, reps=2):
+        """The C autotuner (arp_tune_batched): returns ((workgroups, waves, lds_budget), ms) and sets the shape."""
+        arr, keep = self._c_classes(frames, False)
+        need = self.lib.arp_detect_batched_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(self.cls), self.R)
+        work = torch.empty(need, dtype=torch.uint8, device='cuda')
+        shape = (ctypes.c_int * 3)()
+        ms = ctypes.c_float()
+        _lib.check(self.lib.arp_tune_batched(ctypes.cast(arr, ctypes.c_void_p), len(self.cls), _lib.ptr(self.T),
+                                             self.L, self.q, self.R, self.Q, self.key & MASK, (self.key >> 32) & MASK,
+                                             reps, _lib.ptr(work), work.numel(), shape, ctypes.byref(ms),
+                                             _lib.stream_ptr()), 'tune_batched')
+        self.set_shape(tuple(shape))
+        return tuple(shape), ms.value
+
     def capture(self, frames, with_errors=True):
         """Captures run() once into a HIP graph (after one eager run that also settles launch attributes);
         returns the torch.cuda.CUDAGraph, replayed with .replay()."""

@@ -168,6 +168,42 @@ typedef struct {
 int arp_anneal_arpeggio_batched(const arp_batch_class *classes, int n_classes, int R, uint32_t k0, uint32_t k1,
                                 uint32_t *counter, int workgroups, int waves, void *stream);
 
+/* One class of a mixed batch for the whole-chain entry point: nt users x nt rx, constellation M, B frames. */
+typedef struct arp_batch_frames {
+  const float *H;         /* (B, 2nt, 2nt) real lift, float32 */
+  const float *y;         /* (B, 2nt) */
+  const float *N0;        /* (B) per-frame noise density, or NULL: N0_scalar */
+  float N0_scalar;
+  const uint32_t *inst;   /* (B) Philox instance words (the frames' arrival indices) */
+  const int8_t *ltrue;    /* (B, 2nt) true levels, or NULL */
+  int8_t *lhat;           /* (B, 2nt) decided levels (out) */
+  int32_t *biterr;        /* (B) bit errors (out; needs ltrue) or NULL */
+  int nt, M, B;
+} arp_batch_frames;
+
+/* Workspace bytes arp_detect_batched and arp_tune_batched need for these classes and R anneals per frame (0 for
+ * invalid arguments). */
+size_t arp_detect_batched_workspace_bytes(const arp_batch_frames *classes, int n_classes, int R);
+
+/* The mixed batch's whole step in one call: per class the front end (LMMSE, dDI build, fixed point Q; per-frame N0),
+ * one persistent first-accept anneal over every class (arp_anneal_arpeggio_batched: R anneals of L passes per frame,
+ * schedule T (L) and q in Q units, P = 2, random start, Philox key k0/k1), per class decode and bit errors. Launch
+ * shape: `workgroups` workgroups of `waves` waves, frames per tile from `lds_budget` bytes (arp_tune_batched picks
+ * them). Classes in the order given (largest first packs best), at most 8, nt <= 64. Stream-ordered with no host
+ * synchronisation and no memset, so a caller may capture it in a HIP graph. No reference counterpart (the reference
+ * detects one shape per call, arpeggio/mimo/fxdetect.py:24-43). */
+int arp_detect_batched(const arp_batch_frames *classes, int n_classes, const int32_t *T, int L, int q, int R, float Q,
+                       uint32_t k0, uint32_t k1, int workgroups, int waves, int lds_budget, void *work,
+                       size_t work_bytes, void *stream);
+
+/* Launch-shape autotuner of the persistent anneal: runs the front ends once, then times `reps` persistent launches
+ * for every candidate (LDS budget in {one frame of the largest class, twice that, 80 KiB, 160 KiB}, waves in {4, 8,
+ * 16}, workgroups = CUs x workgroups resident per CU) with HIP events on the stream, and writes the fastest as
+ * best_shape = {workgroups, waves, lds_budget} and its time in ms. Synchronises the stream: not for graph capture. */
+int arp_tune_batched(const arp_batch_frames *classes, int n_classes, const int32_t *T, int L, int q, int R, float Q,
+                     uint32_t k0, uint32_t k1, int reps, void *work, size_t work_bytes, int *best_shape, float *best_ms,
+                     void *stream);
+
 /* ---- Adaptive (early-stop) detection: a short anneal for every frame, a certificate, and a longer anneal for the
  * frames the certificate leaves open (no reference counterpart for the two-stage form; the stopping rule is the
  * reference's first-pass stop t = ||y - H x||^2 / N0 <= Nr + c sqrt(Nr), arpeggio/sim/a56an.py:58-67). */

@@ -149,3 +149,21 @@ def test_ising_ri_argument_errors_without_gpu():
     assert L.arp_ising_ri(None, None, None, 4, 64, 64, 16, 0.15, 4096.0, None, None, None, None, None) == -1
     assert b'null' in L.arp_last_error()
     assert L.arp_ising_ri(None, None, None, 0, 64, 64, 16, 0.15, 4096.0, None, None, None, None, None) == 0
+
+
+def test_batched_chain_argument_errors_without_gpu():
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    arr = (_lib.BatchFrames * 1)()
+    arr[0] = _lib.BatchFrames(None, None, None, 1.0, None, None, None, None, 80, 16, 4)
+    p = ctypes.cast(arr, ctypes.c_void_p)
+    assert L.arp_detect_batched_workspace_bytes(p, 0, 8) == 0
+    assert L.arp_detect_batched(p, 1, None, 64, 0, 8, 4096.0, 0, 0, 512, 8, 66048, None, 0, None) == -1
+    T = (ctypes.c_int * 64)()
+    assert L.arp_detect_batched(p, 1, T, 64, 0, 8, 4096.0, 0, 0, 512, 8, 66048, None, 0, None) == -3
+    arr[0] = _lib.BatchFrames(None, None, None, 1.0, None, None, None, None, 32, 16, 4)
+    assert L.arp_detect_batched(p, 1, T, 64, 0, 8, 4096.0, 0, 0, 512, 8, 66048, None, 0, None) == -1
+    assert b'null' in L.arp_last_error()
+    arr[0] = _lib.BatchFrames(None, None, None, 1.0, None, None, None, None, 32, 16, 0)
+    assert L.arp_detect_batched(p, 1, T, 64, 0, 8, 4096.0, 0, 0, 512, 8, 66048, None, 0, None) == -1
+    assert b'workspace' in L.arp_last_error()

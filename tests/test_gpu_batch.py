@@ -97,3 +97,34 @@ def test_graph_replay_and_autotune(mixed):
     errs = det.bit_errors()
     say('bit errors', errs)
     assert errs >= 0
+
+
+def test_whole_chain_c_entry_equals_python_step():
+    """arp_detect_batched (one C call: front ends, persistent anneal, decodes) gives the Python step's decisions and
+    bit errors; arp_tune_batched returns one of its candidate shapes; a HIP graph of the C call replays equal."""
+    from arpeggio_amd.batch import BatchedDetector, draw_mixed, prepare
+    fr = prepare(draw_mixed(21, 3000, snr_lo=10.0, snr_hi=22.0))
+    det = BatchedDetector([dict(nt=f['nt'], M=f['M'], B=f['l'].shape[0], idx=f['idx']) for f in fr], R=4, passes=32,
+                          key=0x5151)
+    det.run(fr)
+    det.run_c(fr)
+    torch.cuda.synchronize()
+    for c in det.cls:
+        assert (c['lhat_c'] == c['lhat']).all() and (c['biterr_c'] == c['biterr']).all()
+    shape, ms = det.tune_c(fr)
+    assert shape[1] in (4, 8, 16) and shape[0] > 0 and ms > 0
+    det.run(fr)
+    torch.cuda.synchronize()
+    want = [c['lhat'].clone() for c in det.cls]
+    g = torch.cuda.CUDAGraph()
+    det.run_c(fr)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        det.run_c(fr)
+    for _ in range(3):
+        for c in det.cls:
+            c['lhat_c'].fill_(0)
+        g.replay()
+        torch.cuda.synchronize()
+        for c, w in zip(det.cls, want):
+            assert (c['lhat_c'] == w).all()
