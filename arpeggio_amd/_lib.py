@@ -25,7 +25,8 @@ class BatchClass(ctypes.Structure):
     _fields_ = [('Gt', ctypes.c_void_p), ('h', ctypes.c_void_p), ('inst', ctypes.c_void_p), ('T', ctypes.c_void_p),
                 ('out_s', ctypes.c_void_p), ('out_de', ctypes.c_void_p), ('out_st', ctypes.c_void_p),
                 ('D', ctypes.c_int), ('B', ctypes.c_int), ('L', ctypes.c_int), ('q', ctypes.c_int),
-                ('start', ctypes.c_int), ('tile', ctypes.c_int)]
+                ('start', ctypes.c_int), ('tile', ctypes.c_int), ('thr', ctypes.c_void_p), ('out_used', ctypes.c_void_p),
+                ('gate', ctypes.c_int)]
 
 
 class BatchFrames(ctypes.Structure):

@@ -68,9 +68,12 @@ def draw_mixed(seed, B, mix=((16, 0.4), (32, 0.4), (64, 0.2)), mods=(4, 16), snr
 class BatchedDetector:
     """Preallocated buffers for one mixed batch (the class list of draw_mixed: nt, M, B per class, largest first).
     shape = (workgroups, waves, lds_budget_bytes) of the persistent launch; None -> a default, or tune with
-    arpeggio_amd.autotune.tune_batched."""
+    arpeggio_amd.autotune.tune_batched. models: {nt: learned.WarmStartMLP} for the classes with a learned start (the
+    front end also writes their features, the MLP their start thresholds, and the persistent launch starts those
+    anneals from them, with the energy gate `gate`); classes of other sizes start at random."""
 
-    def __init__(self, classes, R=8, passes=64, T0=0.1, T1=0.1, q=0.0, key=0, shape=None, Q=QSCALE):
+    def __init__(self, classes, R=8, passes=64, T0=0.1, T1=0.1, q=0.0, key=0, shape=None, Q=QSCALE, models=None,
+                 gate=True):
         if not 1 <= len(classes) <= 8:
             raise ValueError('a mixed batch holds 1..8 classes')
         self.R, self.P, self.K, self.Q, self.key = R, 2, 1, float(Q), key
@@ -96,6 +99,15 @@ class BatchedDetector:
                 st=torch.empty(B, R, 2, dtype=torch.int64, device='cuda'),
                 lhat=torch.empty(B, D, dtype=torch.int8, device='cuda'),
                 biterr=torch.empty(B, dtype=torch.int32, device='cuda')))
+            m = (models or {}).get(nt)
+            if m is not None:
+                from .learned import feat_ld
+                if m.D != D or m.P != self.P:
+                    raise ValueError('learned start for %d users needs a model with D = %d, P = 2' % (nt, D))
+                self.cls[-1].update(model=m, feat=torch.empty(B, feat_ld(D), dtype=torch.bfloat16, device='cuda'),
+                                    thr=torch.empty(B, self.P * D, dtype=torch.int32, device='cuda'),
+                                    used=torch.empty(B, R, dtype=torch.int8, device='cuda'))
+        self.gate = gate
         self.counter = torch.zeros(1, dtype=torch.int32, device='cuda')
         cus = torch.cuda.get_device_properties(0).multi_processor_count
         self.set_shape(shape or (2 * cus, 8, 66048))
@@ -109,8 +121,11 @@ class BatchedDetector:
         for i, c in enumerate(self.cls):
             tile = max(1, budget // (4 * gram_stride(c['D'])))
             c['tile'] = tile
+            learned = 'thr' in c
             arr[i] = _lib.BatchClass(p(c['Gi']), p(c['hi']), p(c['inst']), p(self.T), p(c['spins']), p(c['de']),
-                                     p(c['st']), c['D'], c['B'], self.L, self.q, 0, tile)
+                                     p(c['st']), c['D'], c['B'], self.L, self.q, 0, tile,
+                                     p(c['thr']) if learned else None, p(c['used']) if learned else None,
+                                     1 if learned and self.gate else 0)
         self._arr = arr
 
     def anneal(self, stream=None):
@@ -127,9 +142,13 @@ class BatchedDetector:
         P_ = _lib.ptr
         for c, f in zip(self.cls, frames):
             n0t, n0s = _n0_arg(f['N0'], c['B'])
+            feat = c.get('feat')
             _lib.check(L.arp_front_end(P_(f['H']), P_(f['y']), P_(n0t), n0s, c['B'], c['Mr'], c['D'], c['M'], 1,
                                        self.Q, P_(c['xg']), P_(c['Gi']), P_(c['hi']), P_(c['scale']), None, None,
-                                       None, None, None, None, None, None, None, 0, None, s), 'front_end')
+                                       None, None, None, None, None, None, P_(feat),
+                                       feat.shape[1] if feat is not None else 0, None, s), 'front_end')
+            if feat is not None:
+                c['model'].thresholds(feat, c['model'].inv_temp, out=c['thr'], stream=stream)
         self.anneal(stream)
         for c, f in zip(self.cls, frames):
             lt = f.get('lt') if with_errors else None
@@ -157,7 +176,10 @@ class BatchedDetector:
 
     def run_c(self, frames, with_errors=True, stream=None):
         """The same step through the single C entry point arp_detect_batched (its own workspace, this detector's launch
-        shape); decisions and bit errors land in each class's 'lhat_c' and 'biterr_c'."""
+        shape); decisions and bit errors land in each class's 'lhat_c' and 'biterr_c'. The C chain starts every anneal
+        at random: classes with a learned start go through run()."""
+        if any('thr' in c for c in self.cls):
+            raise ValueError('run_c: the whole-chain C entry point has no learned start; use run()')
         arr, keep = self._c_classes(frames, with_errors)
         need = self.lib.arp_detect_batched_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(self.cls), self.R)
         if getattr(self, '_work', None) is None or self._work.numel() < need:

@@ -529,29 +529,30 @@ __global__ __launch_bounds__(ARP_BATCHED_MAX_THREADS) void arpeggio_batched_kern
       const int32_t* gs = sG + f * stride;
       const int32_t* hb = C.h + (size_t)b * D;
       const size_t at = (size_t)b * R + r;
+      const uint32_t* thr = C.thr ? C.thr + (size_t)b * 2 * D : nullptr;
       if (cpl == 1) {
         if (full && D == 64)
           arpeggio_anneal<2, 1, 0, true, 64>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
-                                             C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+                                             C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
         else if (full && D == 32)
           arpeggio_anneal<2, 1, 0, true, 32>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
-                                             C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+                                             C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
         else if (full)
           arpeggio_anneal<2, 1, 0, true>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
-                                         C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+                                         C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
         else
           arpeggio_anneal<2, 1, 0, false>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
-                                          C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+                                          C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
       } else {
         if (full && D == 128)
           arpeggio_anneal<2, 2, 0, true, 128>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
-                                              C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+                                              C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
         else if (full)
           arpeggio_anneal<2, 2, 0, true>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
-                                         C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+                                         C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
         else
           arpeggio_anneal<2, 2, 0, false>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
-                                          C.out_s, C.out_de, C.out_st, at, nullptr, 0, nullptr);
+                                          C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
       }
     }
     __syncthreads();  // the tile's LDS is free again

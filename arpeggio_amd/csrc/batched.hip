@@ -88,7 +88,7 @@ int front_and_table(const arp_batch_frames* classes, int n, const int32_t* T, in
     const size_t stride = (size_t)D * D + ARP_WAVE * (D <= ARP_WAVE ? 1 : 2);
     const int tile = (int)(lds_budget / (4 * stride)) > 1 ? (int)(lds_budget / (4 * stride)) : 1;
     tab[i] = arp_batch_class{bufs[i].Gi, bufs[i].hi, c.inst, T, bufs[i].spins, (int64_t*)bufs[i].de,
-                             (int64_t*)bufs[i].st, D, c.B, L, q, 0, tile};
+                             (int64_t*)bufs[i].st, D, c.B, L, q, 0, tile, nullptr, nullptr, 0};
   }
   (void)work_bytes;
   return ARP_OK;

@@ -73,7 +73,7 @@ int hip_status(hipError_t e, const char* what) { return hip_check(e, what); }
 
 extern "C" const char* arp_last_error(void) { return g_err.c_str(); }
 
-extern "C" int arp_abi_version(void) { return 3; }
+extern "C" int arp_abi_version(void) { return 4; }
 
 extern "C" int arp_set_anneal_waves(int waves) {
   if (waves < 0 || waves > 16) return fail(ARP_E_ARG, "set_anneal_waves: %d outside 0..16", waves);
@@ -269,6 +269,9 @@ extern "C" int arp_anneal_arpeggio_batched(const arp_batch_class* classes, int n
     d.q = c.q;
     d.start = c.start;
     d.tile = t;
+    d.thr = c.thr;
+    d.used = c.thr ? c.out_used : nullptr;
+    d.gate = c.gate ? 1 : 0;
     d.first_tile = tiles;
     tiles += (c.B + t - 1) / t;
   }

@@ -34,7 +34,9 @@ struct ArpBatchClass {
   int8_t* out_s;         // (B, R, P D)
   long long* out_de;     // (B, R)
   long long* out_st;     // (B, R, 2)
-  int D, B, L, q, start, tile, first_tile, pad_;
+  int D, B, L, q, start, tile, first_tile, gate;
+  const uint32_t* thr;   // (B, 2 D) learned-start thresholds or nullptr
+  int8_t* used;          // (B, R) kept flags of the energy gate or nullptr
 };
 #define ARP_MAX_CLASSES 8
 struct ArpBatchArgs {

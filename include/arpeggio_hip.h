@@ -147,7 +147,9 @@ int arp_anneal_arpeggio_ws(const int32_t *Gt, const int32_t *h, const int32_t *T
 
 /* One class of a mixed batch (frames of one size D): device buffers with the layouts of arp_anneal_arpeggio, that
  * class's schedule T (L int32), q, start (0 rand, 1 bal) and `tile` = frames per unit of work (their staged Grams must
- * fit the workgroup's LDS together: tile * (D*D + 64*ceil(D/64)) * 4 bytes). */
+ * fit the workgroup's LDS together: tile * (D*D + 64*ceil(D/64)) * 4 bytes). thr (B, 2D) uint32 or NULL: the learned
+ * start of arp_anneal_arpeggio_ws for this class (with the energy gate `gate`, kept flags to out_used (B, R) int8 or
+ * NULL); NULL: the start `start`. */
 typedef struct {
   const int32_t *Gt;
   const int32_t *h;
@@ -157,6 +159,9 @@ typedef struct {
   int64_t *out_de;
   int64_t *out_st;
   int D, B, L, q, start, tile;
+  const uint32_t *thr;
+  int8_t *out_used;
+  int gate;
 } arp_batch_class;
 
 /* Frames of uneven size in one persistent fixed-point Arpeggio launch (P = 2, first accept): `workgroups`

@@ -123,7 +123,7 @@ def test_list_anneal_gram_forms_without_gpu():
     p = ctypes.addressof(one)
     for gt, gh in ((None, None), (p, p)):
         assert L.arp_anneal_arpeggio_list(*args(gt, gh)) == -1 and b'exactly one' in L.arp_last_error()
-    assert L.arp_abi_version() == 3
+    assert L.arp_abi_version() == 4
 
 
 def test_soft_argument_errors_without_gpu():

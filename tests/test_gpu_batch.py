@@ -99,6 +99,50 @@ def test_graph_replay_and_autotune(mixed):
     assert errs >= 0
 
 
+def test_batched_learned_start(mixed, orc):
+    """Classes with a learned start (the trained 32- and 64-user networks) in the persistent launch: every anneal
+    equals arp_anneal_arpeggio_ws on the class's own integers and thresholds, and the oracle's arpeggio_ws on a few
+    frames; the random-start classes of the same launch still equal arp_anneal_arpeggio."""
+    from arpeggio_amd import _lib
+    from arpeggio_amd.batch import BatchedDetector
+    from arpeggio_amd.learned import DEFAULT_WEIGHTS, WEIGHTS_64, WarmStartMLP
+    models = {32: WarmStartMLP.load(DEFAULT_WEIGHTS), 64: WarmStartMLP.load(WEIGHTS_64)}
+    for gate in (True, False):
+        say('batched learned, gate', gate)
+        det = BatchedDetector([dict(nt=f['nt'], M=f['M'], B=f['l'].shape[0], idx=f['idx']) for f in mixed], R=4,
+                              passes=32, key=0xABCDEF12345, models=models, gate=gate)
+        det.run(mixed)
+        torch.cuda.synchronize()
+        T = det.T.cpu().numpy()
+        p = _lib.ptr
+        for c in det.cls:
+            if 'thr' not in c:
+                s, de, st = per_class(det, c)
+                assert torch.equal(s, c['spins']) and torch.equal(de, c['de']) and torch.equal(st, c['st'])
+                continue
+            B, D, R = c['B'], c['D'], det.R
+            s = torch.empty(B, R, 2 * D, dtype=torch.int8, device='cuda')
+            de = torch.empty(B, R, dtype=torch.int64, device='cuda')
+            st = torch.empty(B, R, 2, dtype=torch.int64, device='cuda')
+            used = torch.empty(B, R, dtype=torch.int8, device='cuda')
+            _lib.check(_lib.load().arp_anneal_arpeggio_ws(p(c['Gi']), p(c['hi']), p(det.T), det.L, det.q, 2, R, D, B,
+                                                          p(c['inst']), det.key & 0xFFFFFFFF, det.key >> 32,
+                                                          p(c['thr']), 1 if gate else 0, p(s), p(de), p(st), p(used),
+                                                          _lib.stream_ptr()), 'ws')
+            torch.cuda.synchronize()
+            assert torch.equal(s, c['spins']) and torch.equal(de, c['de']) and torch.equal(st, c['st']), c['nt']
+            assert torch.equal(used, c['used']), c['nt']
+            assert int(c['used'].sum()) > 0 if gate else bool((c['used'] == 1).all())
+            n = min(3, B)
+            os_, ode, ost, oused = orc.arpeggio_ws(c['Gi'][:n].cpu().numpy(), c['hi'][:n].cpu().numpy(), 2, R, T,
+                                                   det.q, c['inst'][:n].cpu().numpy().astype(np.int64) & 0xFFFFFFFF,
+                                                   det.key, c['thr'][:n].cpu().numpy().view(np.uint32), gate=gate)
+            assert (c['spins'][:n].cpu().numpy() == os_).all() and (c['de'][:n].cpu().numpy() == ode).all()
+            assert (c['used'][:n].cpu().numpy() == oused).all()
+    with pytest.raises(ValueError):
+        det.run_c(mixed)
+
+
 def test_whole_chain_c_entry_equals_python_step():
     """arp_detect_batched (one C call: front ends, persistent anneal, decodes) gives the Python step's decisions and
     bit errors; arp_tune_batched returns one of its candidate shapes; a HIP graph of the C call replays equal."""
