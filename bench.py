@@ -318,8 +318,9 @@ def roofline_anneal(t_ms, B, R, L, flips, learned, kernel):
     issue = None
     ip = os.path.join(ROOT, 'profiles', 'arpeggio_anneal_issue.json')
     if os.path.exists(ip):
-        for rec in json.load(open(ip)).get('launches', []):
-            if rec.get('start') == ('learned' if learned else 'rand'):
+        recs = [r for r in json.load(open(ip)).get('launches', []) if r.get('start') == ('learned' if learned else 'rand')]
+        same = [r for r in recs if r.get('copies') == R and r.get('passes') == L]
+        for rec in (same or recs)[:1]:
                 q, qs = rec['valu_quad_cycles_per_anneal_pass'], rec['salu_quad_cycles_per_anneal_pass']
                 avail = t_ms * 1e-3 * SIMDS * CLOCK_HZ
                 issue = dict(bound='instruction issue (vector and scalar)', valu_quad_cycles_per_anneal_pass=q,
@@ -327,13 +328,13 @@ def roofline_anneal(t_ms, B, R, L, flips, learned, kernel):
                              salu_frac=4 * qs * B * R * L / avail,
                              counters_at=dict(frames=rec['frames'], copies=rec['copies'], passes=rec['passes']),
                              source='profiles/arpeggio_anneal_issue.json')
-    tr = None if learned else profile_record('detect_step_traffic.json', 'plain/arpeggio_fx')
+    tr = profile_record('detect_step_traffic.json', 'adaptive/arpeggio_fx_stage1' if learned else 'plain/arpeggio_fx')
     return dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
                 traffic=tr['hbm_bytes_per_launch'] if tr else None, kernel=kernel, algorithmic_bytes=hbm,
                 launch_ms=t_ms, flips_per_anneal=flips,
                 lds=dict(algorithmic_bytes=lds, achieved_GBs=lds / (t_ms * 1e-3) / 1e9, peak_GBs=LDS_PEAK_GBS,
                          frac=lds / (t_ms * 1e-3) / 1e9 / LDS_PEAK_GBS),
-                issue=issue)
+                issue=issue, traffic_source='profiles/detect_step_traffic.json' if tr else None)
 
 
 def feat_len():
