@@ -26,7 +26,7 @@ class BatchClass(ctypes.Structure):
                 ('out_s', ctypes.c_void_p), ('out_de', ctypes.c_void_p), ('out_st', ctypes.c_void_p),
                 ('D', ctypes.c_int), ('B', ctypes.c_int), ('L', ctypes.c_int), ('q', ctypes.c_int),
                 ('start', ctypes.c_int), ('tile', ctypes.c_int), ('thr', ctypes.c_void_p), ('out_used', ctypes.c_void_p),
-                ('gate', ctypes.c_int)]
+                ('gate', ctypes.c_int), ('apw', ctypes.c_int)]
 
 
 class BatchFrames(ctypes.Structure):
@@ -39,6 +39,7 @@ class BatchFrames(ctypes.Structure):
 SIGNATURES = {
     'arp_philox_kat': [_P, _P, _i32, _P],
     'arp_set_anneal_waves': [_i32],
+    'arp_set_anneals_per_wave': [_i32],
     'arp_set_front_end_form': [_i32],
     'arp_anneal_arpeggio': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _i32, _P, _P, _P,
                             _P],

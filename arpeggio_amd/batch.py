@@ -113,9 +113,11 @@ class BatchedDetector:
         self.set_shape(shape or (2 * cus, 8, 66048))
 
     def set_shape(self, shape):
-        """(workgroups, waves, lds budget in bytes): frames per tile = budget // staged frame size (at least 1)."""
-        wg, waves, budget = shape
-        self.shape = (int(wg), int(waves), int(budget))
+        """(workgroups, waves, lds budget in bytes[, anneals per wave]): frames per tile = budget // staged frame size
+        (at least 1); anneals per wave (1 or 2; 0 or absent: the library's choice) applies to the classes of D <= 32."""
+        wg, waves, budget = shape[:3]
+        apw = int(shape[3]) if len(shape) > 3 else 0
+        self.shape = (int(wg), int(waves), int(budget), apw)
         arr = (_lib.BatchClass * len(self.cls))()
         p = lambda t: t.data_ptr()
         for i, c in enumerate(self.cls):
@@ -125,12 +127,12 @@ class BatchedDetector:
             arr[i] = _lib.BatchClass(p(c['Gi']), p(c['hi']), p(c['inst']), p(self.T), p(c['spins']), p(c['de']),
                                      p(c['st']), c['D'], c['B'], self.L, self.q, 0, tile,
                                      p(c['thr']) if learned else None, p(c['used']) if learned else None,
-                                     1 if learned and self.gate else 0)
+                                     1 if learned and self.gate else 0, apw)
         self._arr = arr
 
     def anneal(self, stream=None):
         """The persistent launch alone (on the front end's integers already in the class buffers)."""
-        wg, waves, _ = self.shape
+        wg, waves = self.shape[:2]
         _lib.check(self.lib.arp_anneal_arpeggio_batched(ctypes.cast(self._arr, ctypes.c_void_p), len(self.cls),
                                                         self.R, self.key & MASK,
                                                         (self.key >> 32) & MASK, _lib.ptr(self.counter), wg, waves,
@@ -184,7 +186,7 @@ class BatchedDetector:
         need = self.lib.arp_detect_batched_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(self.cls), self.R)
         if getattr(self, '_work', None) is None or self._work.numel() < need:
             self._work = torch.empty(need, dtype=torch.uint8, device='cuda')
-        wg, waves, budget = self.shape
+        wg, waves, budget = self.shape[:3]
         _lib.check(self.lib.arp_detect_batched(ctypes.cast(arr, ctypes.c_void_p), len(self.cls), _lib.ptr(self.T),
                                                self.L, self.q, self.R, self.Q, self.key & MASK, (self.key >> 32) & MASK,
                                                wg, waves, budget, _lib.ptr(self._work), self._work.numel(),
@@ -192,11 +194,12 @@ class BatchedDetector:
         self._keep = (arr, keep)
 
     def tune_c(self, frames, reps=2):
-        """The C autotuner (arp_tune_batched): returns ((workgroups, waves, lds_budget), ms) and sets the shape."""
+        """The C autotuner (arp_tune_batched): returns ((workgroups, waves, lds_budget, anneals per wave), ms) and
+        sets the shape."""
         arr, keep = self._c_classes(frames, False)
         need = self.lib.arp_detect_batched_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(self.cls), self.R)
         work = torch.empty(need, dtype=torch.uint8, device='cuda')
-        shape = (ctypes.c_int * 3)()
+        shape = (ctypes.c_int * 4)()
         ms = ctypes.c_float()
         _lib.check(self.lib.arp_tune_batched(ctypes.cast(arr, ctypes.c_void_p), len(self.cls), _lib.ptr(self.T),
                                              self.L, self.q, self.R, self.Q, self.key & MASK, (self.key >> 32) & MASK,

@@ -368,10 +368,214 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
   }
 }
 
+// Two anneals per wave for D <= 32 (P = 2, first accept, Philox streams): lanes 0-31 run anneal r of the frame, lanes
+// 32-63 anneal r + 1 (when `two`), each half lane for lane the one-anneal form above with coordinate j = lane & 31 and
+// its own Philox stream word r + half; results equal arpeggio_anneal's for both anneals. The scan commits one flip in
+// each half per step: the two halves' next accepted lanes come from the two 32-bit words of one ballot, each half's
+// lanes read their own Gram column, and a half with no flip left reads the zero padding past the Gram (column D), so
+// its fields stay. A pass takes max(flips of the two) steps, and the per-pass work (Philox block, thresholds, the
+// sign update, the best-state test) is shared by the two anneals.
+template <bool FULL, int DS = 0>
+__device__ __forceinline__ void arpeggio_anneal_pair(const int32_t* __restrict__ G4, const int32_t* __restrict__ hb,
+                                                     const int32_t* __restrict__ T, int L, int q, int D, uint32_t r,
+                                                     bool two, uint32_t inst, uint32_t k0, uint32_t k1, int start,
+                                                     int8_t* __restrict__ out_s, long long* __restrict__ out_de,
+                                                     long long* __restrict__ out_st, size_t t,
+                                                     const uint32_t* __restrict__ thr, int gate,
+                                                     int8_t* __restrict__ used) {
+  constexpr int P = 2;
+  constexpr uint64_t LO = 0xFFFFFFFFull, HI = ~0xFFFFFFFFull;
+  if (DS) D = DS;
+  const int lane = lane_id(), hf = lane >> 5, j = lane & 31;
+  const uint32_t rr = r + (uint32_t)hf;
+  const bool valid = j < D && (hf == 0 || two);
+  const uint64_t vm = ballot(valid);
+  const int32_t dj2 = valid ? G4[j * D + j] >> 1 : 0;
+  uint64_t Sm[P], Rm[P];
+  if (thr) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      uint32_t x = 0u, th = 0u;
+      if (valid) {
+        x = philox_w0(0xFFFFFFFFu, (uint32_t)(p * D + j), rr, inst, k0, k1);
+        th = thr[p * D + j];
+      }
+      Sm[p] = ballot(valid && (uint64_t)x + th >= 0x100000000ull);
+      Rm[p] = ballot(valid && x >= 0x80000000u);
+    }
+  } else if (start == 0) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      Sm[p] = ballot(valid && philox_w0(0xFFFFFFFFu, (uint32_t)(p * D + j), rr, inst, k0, k1) >= 0x80000000u);
+  } else {
+    const uint32_t x = valid ? philox_w0(0xFFFFFFFFu, (uint32_t)j, rr, inst, k0, k1) : 0u;
+    const uint32_t bits = bal_pattern(P, (uint32_t)(((uint64_t)x * bal_count(P)) >> 32));
+#pragma unroll
+    for (int p = 0; p < P; ++p) Sm[p] = ballot(valid && ((bits >> p) & 1u));
+  }
+  // fields w_j = h_j - sum_i G4_ij (m_i / 2): per half the columns with m_i / 2 = +1 (both spins +1) or -1 (both -1)
+  auto fields = [&](const uint64_t (&S)[P]) {
+    int32_t w = valid ? hb[j] : 0;
+    const uint64_t plus = S[0] & S[1];
+    uint64_t nz = plus | (~(S[0] | S[1]) & vm);
+    while (nz) {
+      const uint32_t lo = (uint32_t)nz, hi = (uint32_t)(nz >> 32);
+      const int l0 = lo ? __builtin_ctz(lo) : D, l1 = hi ? __builtin_ctz(hi) : D;
+      const int32_t s0 = lo && ((plus >> l0) & 1ull) ? -1 : 0, s1 = hi && ((plus >> (32 + l1)) & 1ull) ? -1 : 0;
+      if (lo) nz &= ~(1ull << l0);
+      if (hi) nz &= ~(1ull << (32 + l1));
+      const int lc = hf ? l1 : l0;
+      const int32_t sc = hf ? s1 : s0;  // -1: w -= col, 0: w += col
+      w += (G4[lc * D + j] ^ sc) - sc;
+    }
+    return w;
+  };
+  // start energy up to a constant, per half: -(h + w).m summed over the half's lanes (every lane holds its half's)
+  auto energy = [&](const uint64_t (&S)[P], int32_t w) {
+    const int m = 2 * ((int)((S[0] >> lane) & 1ull) + (int)((S[1] >> lane) & 1ull)) - P;
+    long long e = valid ? -(long long)(hb[j] + w) * m : 0;
+#pragma unroll
+    for (int o = 16; o; o >>= 1) e += __shfl_xor(e, o);
+    return e;
+  };
+  int32_t w = fields(Sm);
+  if (thr) {
+    bool take = true;
+    if (gate) {
+      const int32_t wr = fields(Rm);
+      take = energy(Sm, w) <= energy(Rm, wr);
+      const uint64_t keep = ballot(take);
+#pragma unroll
+      for (int p = 0; p < P; ++p) Sm[p] = (Sm[p] & keep) | (Rm[p] & ~keep);
+      w = take ? w : wr;
+    }
+    if (used && j == 0 && (hf == 0 || two)) used[t + hf] = take ? 1 : 0;
+  }
+  int32_t sgp[P], Bs[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    sgp[p] = __builtin_amdgcn_inverse_ballot_w64(Sm[p]) ? 0 : -1;
+    Bs[p] = sgp[p];
+  }
+  long long de0 = 0, de1 = 0, best0 = 1LL << 60, best1 = 1LL << 60;  // per half, de / 4
+  int32_t fc = 0;
+  const PhiloxKeys K = philox_keys_v(k0, k1);
+  uint32_t rw[P][4];
+  for (int kb = 0; kb < L; kb += 4 * P) {
+    const int n4 = kb / (4 * P);
+    int32_t Tb[4 * P];
+#pragma unroll
+    for (int i = 0; i < 4 * P; ++i) Tb[i] = (FULL || kb + i < L) ? T[kb + i] : 0;
+#pragma unroll
+    for (int nm = 0; nm < 4; ++nm) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int k = kb + nm * P + p;
+        if (!FULL && k >= L) break;
+        if (nm == 0) {
+          uint32_t c0 = (uint32_t)n4, c1 = (uint32_t)(p * D + j), c2 = rr, c3 = inst;
+          philox10_kv(c0, c1, c2, c3, K);
+          rw[p][0] = c0;
+          rw[p][1] = c1;
+          rw[p][2] = c2;
+          rw[p][3] = c3;
+        }
+        const int32_t Tk = Tb[nm * P + p];
+        const uint32_t A = 4u * (uint32_t)Tk;
+        const int32_t Bk = q + 2 * Tk;
+        const int32_t tq = (int32_t)(((uint64_t)rw[p][nm] * A + ~0ull) >> 32) - Bk;
+        const int32_t sg = sgp[p], e2 = dj2 - sg;
+        uint64_t rem = vm, fm = 0;
+        int32_t qv = (w ^ sg) + e2;
+        uint64_t m = ballot(qv <= tq) & rem;
+        while (m) {
+          const uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+          int l0 = D, l1 = D;
+          int32_t sl0 = 0, sl1 = 0;
+          uint64_t rn = 0;
+          if (lo) {
+            l0 = __builtin_ctz(lo);
+            de0 += readlane(qv, l0);
+            sl0 = readlane(sg, l0);
+            mask_set(fm, l0);
+            rn = ((~1ull) << l0) & LO;
+          }
+          if (hi) {
+            l1 = __builtin_ctz(hi);
+            de1 += readlane(qv, 32 + l1);
+            sl1 = readlane(sg, 32 + l1);
+            mask_set(fm, 32 + l1);
+            rn |= (~1ull) << (32 + l1);
+          }
+          rem &= rn;
+          const int lc = hf ? l1 : l0;
+          const int32_t sl = hf ? sl1 : sl0;
+          w = ((G4[lc * D + j] ^ sl) + w) - sl;
+          qv = (w ^ sg) + e2;
+          m = ballot(qv <= tq) & rem;
+        }
+        if (fm) {
+          asm volatile("");
+          const bool fl = __builtin_amdgcn_inverse_ballot_w64(fm);
+          sgp[p] ^= fl ? -1 : 0;
+          fc += fl ? 1 : 0;
+        }
+        const bool lt0 = s64_lt(de0, best0), lt1 = s64_lt(de1, best1);
+        if (lt0 || lt1) {
+          asm volatile("");
+          if (lt0) best0 = de0;
+          if (lt1) best1 = de1;
+          const bool u = __builtin_amdgcn_inverse_ballot_w64((lt0 ? LO : 0ull) | (lt1 ? HI : 0ull));
+#pragma unroll
+          for (int pp = 0; pp < P; ++pp) Bs[pp] = u ? sgp[pp] : Bs[pp];
+        }
+      }
+    }
+  }
+  const size_t th = t + (size_t)hf;
+  if (valid) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) out_s[th * (size_t)(P * D) + p * D + j] = Bs[p] ? -1 : 1;
+  }
+  int fl = fc;
+#pragma unroll
+  for (int o = 16; o; o >>= 1) fl += __shfl_xor(fl, o);
+  const int last0 = readlane(fc, D - 1), last1 = readlane(fc, 32 + D - 1);
+  if (j == 0 && (hf == 0 || two)) {
+    out_de[th] = 4 * (hf ? best1 : best0);
+    out_st[2 * th] = fl;
+    out_st[2 * th + 1] = (long long)L + fl - (hf ? last1 : last0);
+  }
+}
+
 // grid.x = B frames (or list slots), block = 64 W threads. LDS: 4 Gt, D*D int32 (+ one wave of padding so that the
 // lanes past D of the last column stay inside the allocation). With a frame list, workgroup i anneals frame list[i]
 // if i < *count and exits otherwise (a uniform exit before any barrier); its outputs go to slot i. Anneal rr of a
 // workgroup uses Philox stream r0 + rr, so a later launch with r0 = R continues a frame's anneals with new streams.
+// Stages frame b's Gram into LDS as 4 Gt (from the int16 Gh when given, else the int32 Gt), with CPL waves of zero
+// padding past it.
+template <int CPL>
+__device__ __forceinline__ void stage_gram(int32_t* __restrict__ sG, const int32_t* __restrict__ Gt,
+                                           const int16_t* __restrict__ Gh, int b, int D) {
+  const int nG = D * D;
+  if (Gh) {
+    const int16_t* g = Gh + (size_t)b * nG;
+    if ((nG & 3) == 0) {  // four entries per 8-byte load (frames start on 8-byte boundaries)
+      for (int x4 = threadIdx.x; x4 < nG / 4; x4 += blockDim.x) {
+        const uint2 w = reinterpret_cast<const uint2*>(g)[x4];
+        reinterpret_cast<int4*>(sG)[x4] = make_int4(4 * (int32_t)(int16_t)(w.x & 0xFFFFu), 4 * ((int32_t)w.x >> 16),
+                                                    4 * (int32_t)(int16_t)(w.y & 0xFFFFu), 4 * ((int32_t)w.y >> 16));
+      }
+      for (int x = nG + threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = 0;
+    } else {
+      for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? 4 * (int32_t)g[x] : 0;
+    }
+  } else {
+    const int32_t* g = Gt + (size_t)b * nG;
+    for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? 4 * g[x] : 0;
+  }
+}
+
 template <int P, int CPL, int MODE, bool FULL, int DS, int RNG>
 __global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt,
                                                                                const int32_t* __restrict__ h,
@@ -391,23 +595,7 @@ __global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) void arpeggio_fx_kernel(c
     if (slot >= *ex.count) return;
     b = ex.list[slot];
   }
-  const int nG = D * D;
-  if (ex.Gh) {
-    const int16_t* g = ex.Gh + (size_t)b * nG;
-    if ((nG & 3) == 0) {  // four entries per 8-byte load (frames start on 8-byte boundaries)
-      for (int x4 = threadIdx.x; x4 < nG / 4; x4 += blockDim.x) {
-        const uint2 w = reinterpret_cast<const uint2*>(g)[x4];
-        reinterpret_cast<int4*>(sG)[x4] = make_int4(4 * (int32_t)(int16_t)(w.x & 0xFFFFu), 4 * ((int32_t)w.x >> 16),
-                                                    4 * (int32_t)(int16_t)(w.y & 0xFFFFu), 4 * ((int32_t)w.y >> 16));
-      }
-      for (int x = nG + threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = 0;
-    } else {
-      for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? 4 * (int32_t)g[x] : 0;
-    }
-  } else {
-    const int32_t* g = Gt + (size_t)b * nG;
-    for (int x = threadIdx.x; x < nG + ARP_WAVE * CPL; x += blockDim.x) sG[x] = x < nG ? 4 * g[x] : 0;
-  }
+  stage_gram<CPL>(sG, Gt, ex.Gh, b, D);
   __syncthreads();
   const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
   const uint32_t in = inst[b];
@@ -416,6 +604,37 @@ __global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) void arpeggio_fx_kernel(c
     arpeggio_anneal<P, CPL, MODE, FULL, DS, RNG>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)(ex.r0 + r), in, k0, k1,
                                                  start, out_s, out_de, out_st, (size_t)slot * R + r, thr, ex.gate,
                                                  ex.used, ex.C);
+}
+
+// The same launch with two anneals per wave (D <= 32, P = 2, first accept): wave w runs the pairs (2i, 2i + 1),
+// i = w, w + W, ...; an odd R leaves the last pair's upper half idle.
+template <bool FULL, int DS>
+__global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) void arpeggio_pair_kernel(const int32_t* __restrict__ Gt,
+                                                                                 const int32_t* __restrict__ h,
+                                                                                 const int32_t* __restrict__ T, int L,
+                                                                                 int q, int R, int D,
+                                                                                 const uint32_t* __restrict__ inst,
+                                                                                 uint32_t k0, uint32_t k1, int start,
+                                                                                 int8_t* __restrict__ out_s,
+                                                                                 long long* __restrict__ out_de,
+                                                                                 long long* __restrict__ out_st,
+                                                                                 AnnealExtra ex) {
+  extern __shared__ int32_t sG[];
+  if (DS) D = DS;
+  const int slot = blockIdx.x;
+  int b = slot;
+  if (ex.list) {
+    if (slot >= *ex.count) return;
+    b = ex.list[slot];
+  }
+  stage_gram<1>(sG, Gt, ex.Gh, b, D);
+  __syncthreads();
+  const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const uint32_t in = inst[b];
+  const uint32_t* thr = ex.thr ? ex.thr + (size_t)b * 2 * D : nullptr;
+  for (int r = 2 * wv; r < R; r += 2 * W)
+    arpeggio_anneal_pair<FULL, DS>(sG, h + (size_t)b * D, T, L, q, D, (uint32_t)(ex.r0 + r), r + 1 < R, in, k0, k1,
+                                   start, out_s, out_de, out_st, (size_t)slot * R + r, thr, ex.gate, ex.used);
 }
 
 template <int P, int CPL, int MODE, bool FULL, int DS, int RNG = 0>
@@ -481,6 +700,21 @@ static hipError_t launch_p(const int32_t* Gt, const int32_t* h, const int32_t* T
 #undef ARP_LANE
     return hipGetLastError();
   }
+  if (P == 2 && mode == 0 && D <= 32 && anneals_per_wave(D) == 2) {
+    // two anneals per wave: half the waves per workgroup for the same anneals
+    const int pw = (R + 1) / 2 < waves ? (R + 1) / 2 : waves;
+    const dim3 pblock(ARP_WAVE * pw);
+    const bool full = L % 8 == 0;
+#define ARP_PAIR(F, DSV)                                                                                             \
+  hipLaunchKernelGGL((arpeggio_pair_kernel<F, DSV>), grid, pblock, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, \
+                     out_s, out_de, out_st, ex)
+    if (D == 32 && full) ARP_PAIR(true, 32);
+    else if (D == 32) ARP_PAIR(false, 32);
+    else if (full) ARP_PAIR(true, 0);
+    else ARP_PAIR(false, 0);
+#undef ARP_PAIR
+    return hipGetLastError();
+  }
   if (cpl == 1) {
     if (mode == 0) launch_pcm<P, 1, 0>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
     else launch_pcm<P, 1, 1>(grid, block, lds, st, Gt, h, T, L, q, R, D, inst, k0, k1, start, out_s, out_de, out_st, ex);
@@ -524,13 +758,25 @@ __global__ __launch_bounds__(ARP_BATCHED_MAX_THREADS) void arpeggio_batched_kern
     }
     __syncthreads();
     const bool full = C.L % 8 == 0;
-    for (int u = wv; u < nf * R; u += W) {
-      const int f = u / R, r = u - f * R, b = f0 + f;
+    const bool pair = C.apw == 2 && D <= 32;  // two anneals per wave: units are pairs (r, r + 1)
+    const int RU = pair ? (R + 1) / 2 : R;
+    for (int u = wv; u < nf * RU; u += W) {
+      const int f = u / RU, r = (u - f * RU) * (pair ? 2 : 1), b = f0 + f;
       const int32_t* gs = sG + f * stride;
       const int32_t* hb = C.h + (size_t)b * D;
       const size_t at = (size_t)b * R + r;
       const uint32_t* thr = C.thr ? C.thr + (size_t)b * 2 * D : nullptr;
-      if (cpl == 1) {
+      if (pair) {
+        if (full && D == 32)
+          arpeggio_anneal_pair<true, 32>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, r + 1 < R, C.inst[b], a.k0, a.k1,
+                                         C.start, C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
+        else if (full)
+          arpeggio_anneal_pair<true>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, r + 1 < R, C.inst[b], a.k0, a.k1, C.start,
+                                     C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
+        else
+          arpeggio_anneal_pair<false>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, r + 1 < R, C.inst[b], a.k0, a.k1,
+                                      C.start, C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);
+      } else if (cpl == 1) {
         if (full && D == 64)
           arpeggio_anneal<2, 1, 0, true, 64>(gs, hb, C.T, C.L, C.q, D, (uint32_t)r, C.inst[b], a.k0, a.k1, C.start,
                                              C.out_s, C.out_de, C.out_st, at, thr, C.gate, C.used);

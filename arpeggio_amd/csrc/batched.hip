@@ -69,7 +69,7 @@ int check_classes(const arp_batch_frames* classes, int n, int R, const int32_t* 
 
 // the per-class front ends and the class table of the persistent launch (tiles from the LDS budget)
 int front_and_table(const arp_batch_frames* classes, int n, const int32_t* T, int L, int q, int R, float Q,
-                    int lds_budget, void* work, size_t work_bytes, hipStream_t st, arp_batch_class* tab,
+                    int lds_budget, int apw, void* work, size_t work_bytes, hipStream_t st, arp_batch_class* tab,
                     ClassBufs* bufs, uint32_t** counter) {
   char* p = static_cast<char*>(work);
   for (int i = 0; i < n; ++i) p = carve(p, classes[i], R, bufs[i]);
@@ -88,7 +88,7 @@ int front_and_table(const arp_batch_frames* classes, int n, const int32_t* T, in
     const size_t stride = (size_t)D * D + ARP_WAVE * (D <= ARP_WAVE ? 1 : 2);
     const int tile = (int)(lds_budget / (4 * stride)) > 1 ? (int)(lds_budget / (4 * stride)) : 1;
     tab[i] = arp_batch_class{bufs[i].Gi, bufs[i].hi, c.inst, T, bufs[i].spins, (int64_t*)bufs[i].de,
-                             (int64_t*)bufs[i].st, D, c.B, L, q, 0, tile, nullptr, nullptr, 0};
+                             (int64_t*)bufs[i].st, D, c.B, L, q, 0, tile, nullptr, nullptr, 0, apw};
   }
   (void)work_bytes;
   return ARP_OK;
@@ -115,7 +115,7 @@ extern "C" int arp_detect_batched(const arp_batch_frames* classes, int n_classes
   arp_batch_class tab[ARP_MAX_CLASSES];
   ClassBufs bufs[ARP_MAX_CLASSES];
   uint32_t* counter = nullptr;
-  rc = front_and_table(classes, n_classes, T, L, q, R, Q, lds_budget, work, work_bytes, st, tab, bufs, &counter);
+  rc = front_and_table(classes, n_classes, T, L, q, R, Q, lds_budget, 0, work, work_bytes, st, tab, bufs, &counter);
   if (rc) return rc;
   rc = arp_anneal_arpeggio_batched(tab, n_classes, R, k0, k1, counter, workgroups, waves, stream);
   if (rc) return rc;
@@ -144,8 +144,10 @@ extern "C" int arp_tune_batched(const arp_batch_frames* classes, int n_classes, 
                                               hipSuccess)
     return arp::set_error(ARP_E_HIP, "tune_batched: device query failed");
   size_t need = 0;
+  bool small = false;  // a class of D <= 32: anneals per wave 1 and 2 are both candidates
   for (int i = 0; i < n_classes; ++i) {
     const size_t D = 2 * (size_t)classes[i].nt;
+    small = small || D <= 32;
     const size_t s = 4 * (D * D + ARP_WAVE * (D <= ARP_WAVE ? 1 : 2));
     need = need > s ? need : s;
   }
@@ -158,7 +160,7 @@ extern "C" int arp_tune_batched(const arp_batch_frames* classes, int n_classes, 
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return arp::set_error(ARP_E_HIP, "tune_batched: event creation failed");
   float best = -1.f;
-  int bw = 0, bv = 0, bb = 0;
+  int bw = 0, bv = 0, bb = 0, ba = 1;
   int tried[4] = {-1, -1, -1, -1};
   for (int k = 0; k < 4 && !rc; ++k) {
     const int budget = budgets[k];
@@ -166,10 +168,12 @@ extern "C" int arp_tune_batched(const arp_batch_frames* classes, int n_classes, 
     for (int j = 0; j < k; ++j) seen = seen || tried[j] == budget;
     tried[k] = budget;
     if (seen) continue;
-    rc = front_and_table(classes, n_classes, T, L, q, R, Q, budget, work, work_bytes, st, tab, bufs, &counter);
+    rc = front_and_table(classes, n_classes, T, L, q, R, Q, budget, 1, work, work_bytes, st, tab, bufs, &counter);
     const int per_cu = kLdsPerCu / budget > 1 ? kLdsPerCu / budget : 1;
-    for (int waves : {4, 8, 16}) {
+    for (int cand = 0; cand < (small ? 6 : 3); ++cand) {
       if (rc) break;
+      const int waves = 4 << (cand % 3), apw = 1 + cand / 3;
+      for (int i = 0; i < n_classes; ++i) tab[i].apw = apw;
       const int wg = cus * per_cu;
       rc = arp_anneal_arpeggio_batched(tab, n_classes, R, k0, k1, counter, wg, waves, stream);  // warm-up
       (void)hipEventRecord(e0, st);
@@ -189,6 +193,7 @@ extern "C" int arp_tune_batched(const arp_batch_frames* classes, int n_classes, 
         bw = wg;
         bv = waves;
         bb = budget;
+        ba = apw;
       }
     }
   }
@@ -198,6 +203,7 @@ extern "C" int arp_tune_batched(const arp_batch_frames* classes, int n_classes, 
   best_shape[0] = bw;
   best_shape[1] = bv;
   best_shape[2] = bb;
+  best_shape[3] = ba;
   if (best_ms) *best_ms = best;
   return arp::set_error(0, "");
 }

@@ -63,6 +63,15 @@ static int arpeggio_waves(int R, int D) {
   return R < w ? R : w;
 }
 
+// Anneals per wave64 of the first-accept launches for D <= 32 (two copies): 0 = the library's choice (kAutoApw), 1 or
+// 2. Frames of more than 32 coordinates always run one anneal per wave.
+static thread_local int g_apw = 0;
+static constexpr int kAutoApw = 1;
+
+namespace arp {
+int anneals_per_wave(int D) { return D <= 32 ? (g_apw ? g_apw : kAutoApw) : 1; }
+}  // namespace arp
+
 // A workgroup of `waves` wave64s must fit the kernel's compiled bound.
 static bool block_fits(int waves, int max_threads) { return waves >= 1 && ARP_WAVE * waves <= max_threads; }
 
@@ -73,11 +82,17 @@ int hip_status(hipError_t e, const char* what) { return hip_check(e, what); }
 
 extern "C" const char* arp_last_error(void) { return g_err.c_str(); }
 
-extern "C" int arp_abi_version(void) { return 4; }
+extern "C" int arp_abi_version(void) { return 5; }
 
 extern "C" int arp_set_anneal_waves(int waves) {
   if (waves < 0 || waves > 16) return fail(ARP_E_ARG, "set_anneal_waves: %d outside 0..16", waves);
   g_waves_override = waves;
+  return ok();
+}
+
+extern "C" int arp_set_anneals_per_wave(int apw) {
+  if (apw < 0 || apw > 2) return fail(ARP_E_ARG, "set_anneals_per_wave: %d outside 0..2", apw);
+  g_apw = apw;
   return ok();
 }
 
@@ -247,6 +262,7 @@ extern "C" int arp_anneal_arpeggio_batched(const arp_batch_class* classes, int n
   for (int i = 0; i < n_classes; ++i) {
     const arp_batch_class& c = classes[i];
     if (c.D < 1 || c.D > 128) return fail(ARP_E_RANGE, "arpeggio_batched: class %d D = %d outside 1..128", i, c.D);
+    if (c.apw < 0 || c.apw > 2) return fail(ARP_E_ARG, "arpeggio_batched: class %d apw = %d outside 0..2", i, c.apw);
     if (c.B < 0 || c.L < 1 || c.tile < 1 || (c.start != 0 && c.start != 1))
       return fail(ARP_E_ARG, "arpeggio_batched: class %d B = %d, L = %d, tile = %d, start = %d", i, c.B, c.L, c.tile,
                   c.start);
@@ -272,6 +288,7 @@ extern "C" int arp_anneal_arpeggio_batched(const arp_batch_class* classes, int n
     d.thr = c.thr;
     d.used = c.thr ? c.out_used : nullptr;
     d.gate = c.gate ? 1 : 0;
+    d.apw = c.D <= 32 ? (c.apw ? c.apw : arp::anneals_per_wave(c.D)) : 1;
     d.first_tile = tiles;
     tiles += (c.B + t - 1) / t;
   }

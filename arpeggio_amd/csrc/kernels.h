@@ -37,6 +37,7 @@ struct ArpBatchClass {
   int D, B, L, q, start, tile, first_tile, gate;
   const uint32_t* thr;   // (B, 2 D) learned-start thresholds or nullptr
   int8_t* used;          // (B, R) kept flags of the energy gate or nullptr
+  int apw;               // anneals per wave (2: pairs, for D <= 32)
 };
 #define ARP_MAX_CLASSES 8
 struct ArpBatchArgs {
@@ -44,6 +45,10 @@ struct ArpBatchArgs {
   int n_classes, n_tiles, R, pad_;
   uint32_t k0, k1;
 };
+
+// Anneals per wave64 of the first-accept launches (two copies) for frames of D coordinates: 2 packs two anneals into
+// a wave (D <= 32 only), 1 runs one; set per host thread by arp_set_anneals_per_wave.
+int anneals_per_wave(int D);
 
 // number of learned-start features for D coordinates (before padding)
 __host__ __device__ inline int feat_count(int D) { return 4 * D + 1; }

@@ -461,8 +461,10 @@ def batched(args):
     return dict(value=B * args.steps / dt, unit='detections/s', ms_per_step=dt / args.steps * 1e3,
                 gpu_ms_per_step=e0.elapsed_time(e1) / args.steps, frames=B,
                 classes=[dict(users=c['nt'], M=c['M'], frames=c['B'], tile=c['tile']) for c in det.cls],
-                shape=dict(workgroups=shape[0], waves=shape[1], lds_budget=shape[2]), anneal_ms=best_ms,
-                autotune=[dict(ms=ms, workgroups=sh[0], waves=sh[1], lds_budget=sh[2]) for ms, sh in tried],
+                shape=dict(workgroups=shape[0], waves=shape[1], lds_budget=shape[2], anneals_per_wave=shape[3]),
+                anneal_ms=best_ms,
+                autotune=[dict(ms=ms, workgroups=sh[0], waves=sh[1], lds_budget=sh[2], anneals_per_wave=sh[3])
+                          for ms, sh in tried],
                 ber=det.bit_errors() / nbits(fr), R=8, passes=64, snr_db=[8.0, 24.0], graph=True)
 
 

@@ -37,9 +37,16 @@ int arp_abi_version(void);
  * A tuning knob with no reference counterpart (the reference launches 128-thread blocks, arpeggiofx.py:39-42). */
 int arp_set_anneal_waves(int waves);
 
+/* Anneals per wave64 of the two-copy first-accept launches for frames of D <= 32 (arp_anneal_arpeggio, _ws, _list,
+ * _batched, the adaptive chains) called afterwards on this thread: 2 runs two anneals per wave (lanes 0-31 anneal r,
+ * lanes 32-63 anneal r + 1), 1 one anneal per wave (half the lanes idle at D = 32), 0 the library's choice. Results
+ * are identical either way. The reference runs one anneal per thread (arpeggio_fx.cu:39-42); this packing has no
+ * counterpart there. */
+int arp_set_anneals_per_wave(int apw);
+
 /* Front-end schedule for D, Mr <= 64 (no counterpart in the reference, which has one float front end): 0 (default)
  * the matrix-core block sweep, 1 the register sweep. Both compute the same quantities; the results agree to float32
- * rounding. Process-wide, like arp_set_anneal_waves. */
+ * rounding. Process-wide. */
 int arp_set_front_end_form(int form);
 
 /* Philox4x32-10 known-answer kernel: in (n, 6) uint32 rows (counter[4], key[2]) -> out (n, 4).
@@ -149,7 +156,8 @@ int arp_anneal_arpeggio_ws(const int32_t *Gt, const int32_t *h, const int32_t *T
  * class's schedule T (L int32), q, start (0 rand, 1 bal) and `tile` = frames per unit of work (their staged Grams must
  * fit the workgroup's LDS together: tile * (D*D + 64*ceil(D/64)) * 4 bytes). thr (B, 2D) uint32 or NULL: the learned
  * start of arp_anneal_arpeggio_ws for this class (with the energy gate `gate`, kept flags to out_used (B, R) int8 or
- * NULL); NULL: the start `start`. */
+ * NULL); NULL: the start `start`. apw: anneals per wave64 for a class of D <= 32 (2: two per wave, 1: one, 0: the
+ * thread's arp_set_anneals_per_wave choice); ignored above D = 32. */
 typedef struct {
   const int32_t *Gt;
   const int32_t *h;
@@ -162,6 +170,7 @@ typedef struct {
   const uint32_t *thr;
   int8_t *out_used;
   int gate;
+  int apw;
 } arp_batch_class;
 
 /* Frames of uneven size in one persistent fixed-point Arpeggio launch (P = 2, first accept): `workgroups`
@@ -194,7 +203,8 @@ size_t arp_detect_batched_workspace_bytes(const arp_batch_frames *classes, int n
  * one persistent first-accept anneal over every class (arp_anneal_arpeggio_batched: R anneals of L passes per frame,
  * schedule T (L) and q in Q units, P = 2, random start, Philox key k0/k1), per class decode and bit errors. Launch
  * shape: `workgroups` workgroups of `waves` waves, frames per tile from `lds_budget` bytes (arp_tune_batched picks
- * them). Classes in the order given (largest first packs best), at most 8, nt <= 64. Stream-ordered with no host
+ * them); anneals per wave of the classes of D <= 32: the thread's arp_set_anneals_per_wave choice. Classes in the
+ * order given (largest first packs best), at most 8, nt <= 64. Stream-ordered with no host
  * synchronisation and no memset, so a caller may capture it in a HIP graph. No reference counterpart (the reference
  * detects one shape per call, arpeggio/mimo/fxdetect.py:24-43). */
 int arp_detect_batched(const arp_batch_frames *classes, int n_classes, const int32_t *T, int L, int q, int R, float Q,
@@ -203,8 +213,9 @@ int arp_detect_batched(const arp_batch_frames *classes, int n_classes, const int
 
 /* Launch-shape autotuner of the persistent anneal: runs the front ends once, then times `reps` persistent launches
  * for every candidate (LDS budget in {one frame of the largest class, twice that, 80 KiB, 160 KiB}, waves in {4, 8,
- * 16}, workgroups = CUs x workgroups resident per CU) with HIP events on the stream, and writes the fastest as
- * best_shape = {workgroups, waves, lds_budget} and its time in ms. Synchronises the stream: not for graph capture. */
+ * 16}, workgroups = CUs x workgroups resident per CU, anneals per wave in {1, 2} when a class has D <= 32) with HIP
+ * events on the stream, and writes the fastest as best_shape = {workgroups, waves, lds_budget, anneals_per_wave}
+ * (4 ints) and its time in ms. Synchronises the stream: not for graph capture. */
 int arp_tune_batched(const arp_batch_frames *classes, int n_classes, const int32_t *T, int L, int q, int R, float Q,
                      uint32_t k0, uint32_t k1, int reps, void *work, size_t work_bytes, int *best_shape, float *best_ms,
                      void *stream);

@@ -51,6 +51,18 @@ def test_batched_argument_errors_without_gpu():
     assert L.arp_anneal_arpeggio_batched(ct.cast(arr, ct.c_void_p), 0, 4, 0, 0, ct.c_void_p(16), 4, 4, None) == -1
     arr[0] = _lib.BatchClass(None, None, None, None, None, None, None, 64, 0, 8, 0, 0, 1)
     assert L.arp_anneal_arpeggio_batched(ct.cast(arr, ct.c_void_p), 1, 4, 0, 0, ct.c_void_p(16), 4, 4, None) == 0
+    arr[0].apw = 3
+    rc = L.arp_anneal_arpeggio_batched(ct.cast(arr, ct.c_void_p), 1, 4, 0, 0, ct.c_void_p(16), 4, 4, None)
+    assert rc == -1 and b'apw = 3' in L.arp_last_error()
+
+
+def test_anneals_per_wave_setting_without_gpu():
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    assert L.arp_set_anneals_per_wave(3) == -1 and b'outside 0..2' in L.arp_last_error()
+    assert L.arp_set_anneals_per_wave(-1) == -1
+    for a in (2, 1, 0):
+        assert L.arp_set_anneals_per_wave(a) == 0
 
 
 def test_oversized_workgroup_refused_without_gpu():
@@ -123,7 +135,7 @@ def test_list_anneal_gram_forms_without_gpu():
     p = ctypes.addressof(one)
     for gt, gh in ((None, None), (p, p)):
         assert L.arp_anneal_arpeggio_list(*args(gt, gh)) == -1 and b'exactly one' in L.arp_last_error()
-    assert L.arp_abi_version() == 4
+    assert L.arp_abi_version() == 5
 
 
 def test_soft_argument_errors_without_gpu():

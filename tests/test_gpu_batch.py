@@ -156,7 +156,7 @@ def test_whole_chain_c_entry_equals_python_step():
     for c in det.cls:
         assert (c['lhat_c'] == c['lhat']).all() and (c['biterr_c'] == c['biterr']).all()
     shape, ms = det.tune_c(fr)
-    assert shape[1] in (4, 8, 16) and shape[0] > 0 and ms > 0
+    assert shape[1] in (4, 8, 16) and shape[0] > 0 and shape[3] in (1, 2) and ms > 0
     det.run(fr)
     torch.cuda.synchronize()
     want = [c['lhat'].clone() for c in det.cls]
