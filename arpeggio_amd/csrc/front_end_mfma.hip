@@ -12,12 +12,15 @@
 //      parts are one symmetric rank-4 update A <- A - (V A_SS^-1) V^T on every tile, except that the diagonal of A_SS
 //      comes out 2 too high; no later block reads those entries, so the 2 comes off once, at the end. The update is
 //      one MFMA per upper tile (A operand V A_SS^-1, B operand V^T); the panel (rows S, every column) goes through a
-//      1 KiB LDS buffer, and the 4x4 inverse of A_SS is swept in registers by every lane. The right-hand side c rides
-//      along as an extra column (c <- c - V A_SS^-1 c_S), so it ends as z = A^-1 c.
+//      1 KiB LDS buffer. A_SS (symmetric positive definite: the Schur complement of the pivots swept so far) is
+//      factored as L D L^T in registers, and each lane solves for the one column of A_SS^-1 it feeds to the update
+//      (its group's) and for A_SS^-1 c_S. The right-hand side c rides along as an extra column
+//      (c <- c - V A_SS^-1 c_S), so it ends as z = A^-1 c.
 //   3. mu_i = 1 - sigma (A^-1)_ii, x_g = nearest level of z / mu (ties to the lower level).
 //   4. h = 2 H^T (y - H x_g) (mimo.ising_delta_at) as 2 (c - G x_g), one row of the LDS copy per lane;
 //      scale = max(max|h|, max|G|), clamped at 1e-12; G_int = rne(G / scale * Q), h_int = rne(h / scale * Q),
-//      G written row-major: 16-byte loads of four rows' entries from the LDS copy, whole rows per store instruction.
+//      G written row-major: 16-byte loads of four rows' entries from the LDS copy, whole rows per store instruction;
+//      rne by the bits of x + 1.5 2^23 (exact for |x| < 2^22), so two int16 entries pack with one byte permute.
 // Reference semantics: mimo.mmse / _mu / quantize_levels (arpeggio/mimo/mimo.py:86-109, 176-182), mimo.ising_delta_at
 // (mimo.py:506-523), chordfx.to_fixed (arpeggio/mimo/chordfx.py:60-76), chord_ref.compress
 // (arpeggio/sim/x3model/chord_ref.py:47-59). The expanded Chord inputs (J, Jf) are the register-sweep kernel's.
@@ -144,6 +147,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DP >= 64 ? 3
     for (int v = 0; v < 4; ++v)
       if (4 * grp + v == col) T[tile<nT>(I, I)][v] = nT * col + I < D ? T[tile<nT>(I, I)][v] + sigma : 1.f;
 
+  float eg[4];  // e_grp: this lane's right-hand side of the pivot-block solve
+#pragma unroll
+  for (int t = 0; t < 4; ++t) eg[t] = grp == t ? 1.f : 0.f;
   // 2. Block sweep over the pivot sets {k0 .. k0 + 3} of the tile order (padding pivots included: identity rows
   // sweep to themselves); the panel buffer alternates between two halves, so a block's stores never meet the
   // previous block's loads.
@@ -178,49 +184,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DP >= 64 ? 3
     float cS[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) cS[t] = lane_value(c[J0], 4 * g0 + t);
-    // m = A_SS swept in place on its upper triangle (m[i][j], i <= j): afterwards m = -A_SS^-1
-    float m[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = i; j < 4; ++j) m[i][j] = S4[j][i];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const float piv = __builtin_amdgcn_rcpf(m[p][p]);
-      float rr[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (j != p) rr[j] = (p < j ? m[p][j] : m[j][p]) * piv;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = i; j < 4; ++j)
-          if (i != p && j != p) m[i][j] = __builtin_fmaf(-(i < p ? m[i][p] : m[p][i]), rr[j], m[i][j]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (j != p) (p < j ? m[p][j] : m[j][p]) = rr[j];
-      m[p][p] = -piv;
-    }
+    // A_SS = L D L^T (A_SS is the Schur complement of the pivots swept so far: symmetric positive definite), then two
+    // solves: x = A_SS^-1 e_grp (column grp of the inverse, a lane's own right-hand side) and qv = -A_SS^-1 c_S.
+    const float a00 = S4[0][0], a01 = S4[1][0], a02 = S4[2][0], a03 = S4[3][0];
+    const float a11 = S4[1][1], a12 = S4[2][1], a13 = S4[3][1], a22 = S4[2][2], a23 = S4[3][2], a33 = S4[3][3];
+    const float r0 = __builtin_amdgcn_rcpf(a00);
+    const float l10 = a01 * r0, l20 = a02 * r0, l30 = a03 * r0;
+    const float r1 = __builtin_amdgcn_rcpf(__builtin_fmaf(-l10, a01, a11));
+    const float t21 = __builtin_fmaf(-l20, a01, a12), t31 = __builtin_fmaf(-l30, a01, a13);
+    const float l21 = t21 * r1, l31 = t31 * r1;
+    const float r2 = __builtin_amdgcn_rcpf(__builtin_fmaf(-l21, t21, __builtin_fmaf(-l20, a02, a22)));
+    const float t32 = __builtin_fmaf(-l31, t21, __builtin_fmaf(-l30, a02, a23));
+    const float l32 = t32 * r2;
+    const float r3 =
+        __builtin_amdgcn_rcpf(__builtin_fmaf(-l32, t32, __builtin_fmaf(-l31, t31, __builtin_fmaf(-l30, a03, a33))));
+    auto solve = [&](float b0, float b1, float b2, float b3, float (&x)[4]) {
+      const float y1 = __builtin_fmaf(-l10, b0, b1);
+      const float y2 = __builtin_fmaf(-l21, y1, __builtin_fmaf(-l20, b0, b2));
+      const float y3 = __builtin_fmaf(-l32, y2, __builtin_fmaf(-l31, y1, __builtin_fmaf(-l30, b0, b3)));
+      x[3] = y3 * r3;
+      x[2] = __builtin_fmaf(-l32, x[3], y2 * r2);
+      x[1] = __builtin_fmaf(-l31, x[3], __builtin_fmaf(-l21, x[2], y1 * r1));
+      x[0] = __builtin_fmaf(-l30, x[3], __builtin_fmaf(-l20, x[2], __builtin_fmaf(-l10, x[1], b0 * r0)));
+    };
     // V = A_.S - E: minus one where the row is the pivot itself
 #pragma unroll
     for (int t = 0; t < 4; ++t) Vr[J0][t] -= col == 4 * g0 + t ? 1.f : 0.f;
     vt[J0] -= col == 4 * g0 + grp ? 1.f : 0.f;
-    // mg = column grp of m (= -A_SS^-1), qv = m c_S (= -A_SS^-1 c_S)
     float mg[4], qv[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const float m0 = t < 1 ? m[t][0] : m[0][t], m1 = t < 2 ? m[t][1] : m[1][t];
-      const float m2 = t < 3 ? m[t][2] : m[2][t], m3 = m[t][3];
-      mg[t] = pick4(floatx4{m0, m1, m2, m3}, grp);
-      qv[t] = __builtin_fmaf(m3, cS[3], __builtin_fmaf(m2, cS[2], __builtin_fmaf(m1, cS[1], m0 * cS[0])));
-    }
+    solve(eg[0], eg[1], eg[2], eg[3], mg);   // mg = A_SS^-1 e_grp; the update takes -mg
+    solve(cS[0], cS[1], cS[2], cS[3], qv);   // qv = A_SS^-1 c_S; c takes -V qv
     float nu[nT];  // -(V A_SS^-1)[16 I + col][grp]: the A operand of the update
 #pragma unroll
     for (int I = 0; I < nT; ++I) {
-      nu[I] = __builtin_fmaf(Vr[I][3], mg[3], __builtin_fmaf(Vr[I][2], mg[2],
-                             __builtin_fmaf(Vr[I][1], mg[1], Vr[I][0] * mg[0])));
-      c[I] = __builtin_fmaf(Vr[I][3], qv[3], __builtin_fmaf(Vr[I][2], qv[2],
-                            __builtin_fmaf(Vr[I][1], qv[1], __builtin_fmaf(Vr[I][0], qv[0], c[I]))));
+      nu[I] = __builtin_fmaf(-Vr[I][3], mg[3], __builtin_fmaf(-Vr[I][2], mg[2],
+                             __builtin_fmaf(-Vr[I][1], mg[1], -Vr[I][0] * mg[0])));
+      c[I] = __builtin_fmaf(-Vr[I][3], qv[3], __builtin_fmaf(-Vr[I][2], qv[2],
+                            __builtin_fmaf(-Vr[I][1], qv[1], __builtin_fmaf(-Vr[I][0], qv[0], c[I]))));
     }
 #pragma unroll
     for (int I = 0; I < nT; ++I)
@@ -348,34 +348,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DP >= 64 ? 3
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int i = nT * (4 * grp + k) + I, j0 = nT * col;
-        float gs[4];
-        int32_t gq[4];
+        const int at = (nT * k + I) * DP;
+        // rne(g invQ) from the bits of g invQ + 1.5 2^23 (|g invQ| <= Q <= 2^14: the sum's low mantissa bits hold the
+        // rounded integer, ties to even as rintf), so an int16 pair is one byte permute
+        uint32_t gb[4];
 #pragma unroll
         for (int J = 0; J < 4; ++J) {
           const float g = J < nT ? blk[J % nT][k] : 0.f;
-          gq[J] = (int32_t)rintf(g * invQ);
-          gs[J] = gfl ? g * inv : 0.f;
+          gb[J] = __builtin_bit_cast(uint32_t, g * invQ + 12582912.f) - 0x4B400000u;
         }
-        const int at = (nT * k + I) * DP;
         if (full && nT == 4) {
-          if (gil) *reinterpret_cast<int4*>(gil + at) = make_int4(gq[0], gq[1], gq[2], gq[3]);
-          if (ghl) {
-            const uint32_t lo = (uint32_t)(uint16_t)gq[0] | ((uint32_t)(uint16_t)gq[1] << 16);
-            const uint32_t hi = (uint32_t)(uint16_t)gq[2] | ((uint32_t)(uint16_t)gq[3] << 16);
-            *reinterpret_cast<uint2*>(ghl + at) = make_uint2(lo, hi);
-          }
-          if (gfl) *reinterpret_cast<floatx4*>(gfl + at) = floatx4{gs[0], gs[1], gs[2], gs[3]};
+          if (gil) *reinterpret_cast<int4*>(gil + at) = make_int4((int)gb[0], (int)gb[1], (int)gb[2], (int)gb[3]);
+          if (ghl)
+            *reinterpret_cast<uint2*>(ghl + at) =
+                make_uint2(__builtin_amdgcn_perm(gb[1], gb[0], 0x05040100u), __builtin_amdgcn_perm(gb[3], gb[2], 0x05040100u));
+          if (gfl) *reinterpret_cast<floatx4*>(gfl + at) = floatx4{blk[0][k] * inv, blk[1][k] * inv, blk[2][k] * inv,
+                                                                 blk[3][k] * inv};
         } else if (full && nT == 2) {
-          if (gil) *reinterpret_cast<int2*>(gil + at) = make_int2(gq[0], gq[1]);
-          if (ghl) *reinterpret_cast<uint32_t*>(ghl + at) = (uint32_t)(uint16_t)gq[0] | ((uint32_t)(uint16_t)gq[1] << 16);
-          if (gfl) *reinterpret_cast<float2*>(gfl + at) = make_float2(gs[0], gs[1]);
+          if (gil) *reinterpret_cast<int2*>(gil + at) = make_int2((int)gb[0], (int)gb[1]);
+          if (ghl) *reinterpret_cast<uint32_t*>(ghl + at) = __builtin_amdgcn_perm(gb[1], gb[0], 0x05040100u);
+          if (gfl) *reinterpret_cast<float2*>(gfl + at) = make_float2(blk[0][k] * inv, blk[1 % nT][k] * inv);
         } else if (i < D) {
 #pragma unroll
           for (int J = 0; J < nT; ++J) {
             if (j0 + J >= D) continue;
             const size_t at = (size_t)i * D + j0 + J;
-            if (Gib) Gib[at] = gq[J];
-            if (Ghb) Ghb[at] = (int16_t)gq[J];
+            if (Gib) Gib[at] = (int32_t)gb[J];
+            if (Ghb) Ghb[at] = (int16_t)gb[J];
             if (Gfb) Gfb[at] = blk[J][k] * inv;
           }
         }
