@@ -97,7 +97,7 @@ extern "C" int arp_set_anneals_per_wave(int apw) {
 }
 
 extern "C" int arp_set_front_end_form(int form) {
-  if (form != 0 && form != 1) return fail(ARP_E_ARG, "set_front_end_form: %d, expected 0 or 1", form);
+  if (form < 0 || form > 2) return fail(ARP_E_ARG, "set_front_end_form: %d, expected 0, 1 or 2", form);
   arp::set_front_end_form(form);
   return ok();
 }

@@ -283,7 +283,10 @@ __device__ __forceinline__ float wave_sum(float v) {
 // j forms coordinate j of every candidate into LDS (over the staged H, once it is in registers); each candidate's
 // residual rows are dot products over the row (no bound tests) and meet in wave_sum. Same selection rule and extras
 // as decode_w64_kernel.
-template <int D>
+// LIFT: H is the real lift of a complex channel, [[Re, -Im], [Im, Re]] (n = D / 2): only its first n columns
+// C1 = [Re; Im] are read, and lane r rebuilds its row as [C1[r], -C1[r + n]] (r < n) or [C1[r], C1[r - n]]: the same
+// values as the stored row, so the distances are the full-H ones bit for bit, from half the bytes.
+template <int D, bool LIFT>
 __global__ __launch_bounds__(64) void decode_dd_kernel(const float* __restrict__ H, const float* __restrict__ y,
                                                        const int8_t* __restrict__ xg,
                                                        const int8_t* __restrict__ spins, int R, int M,
@@ -300,27 +303,42 @@ __global__ __launch_bounds__(64) void decode_dd_kernel(const float* __restrict__
   // read before any lane writes best_d, which may alias prior_d
   const float pdist = ex.prior_d ? ex.prior_d[b] : 0.f;
   const int C = R + 1;
-  constexpr int HS = D + 4;   // LDS row stride of the staged H, in words
-  float* Lc = sm;             // C x D candidate levels (after H has left the LDS)
+  constexpr int W = LIFT ? D / 2 : D;   // staged columns per row
+  constexpr int HS = W + 4;             // LDS row stride of the staged H, in words
+  float* Lc = sm;                       // C x D candidate levels (after H has left the LDS)
   const float* Hb = H + (size_t)b * D * D;
   const bool row = lane < D;
-  constexpr int NV = D * D / 4 / 64;   // 16-byte pieces per lane
+  constexpr int NV = D * W / 4 / 64;    // 16-byte pieces per lane
   floatx4 stage[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) stage[k] = reinterpret_cast<const floatx4*>(Hb)[64 * k + lane];
+  for (int k = 0; k < NV; ++k) {
+    const int f = 64 * k + lane, r = f / (W / 4), c4 = f % (W / 4);
+    stage[k] = *reinterpret_cast<const floatx4*>(Hb + (size_t)r * D + 4 * c4);
+  }
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    const int f = 64 * k + lane, r = f / (D / 4), c4 = f % (D / 4);
+    const int f = 64 * k + lane, r = f / (W / 4), c4 = f % (W / 4);
     *reinterpret_cast<floatx4*>(sm + r * HS + 4 * c4) = stage[k];
   }
   __syncthreads();
   float hr[D];
 #pragma unroll
-  for (int q = 0; q < D / 4; ++q) {
+  for (int q = 0; q < W / 4; ++q) {
     floatx4 t = {0.f, 0.f, 0.f, 0.f};
     if (row) t = *reinterpret_cast<const floatx4*>(sm + lane * HS + 4 * q);
 #pragma unroll
     for (int e = 0; e < 4; ++e) hr[4 * q + e] = t[e];
+  }
+  if constexpr (LIFT) {
+    const int r2 = lane < W ? lane + W : lane - W;
+    const float sg = lane < W ? -1.f : 1.f;
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+      floatx4 t = {0.f, 0.f, 0.f, 0.f};
+      if (row) t = *reinterpret_cast<const floatx4*>(sm + r2 * HS + 4 * q);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) hr[W + 4 * q + e] = sg * t[e];
+    }
   }
   const float yi = row ? y[(size_t)b * D + lane] : 0.f;
   const float lmax = M == 4 ? 1.f : 3.f;
@@ -489,13 +507,16 @@ hipError_t launch_decode_ex(const float* H, const float* y, const int8_t* xg, co
                             float* dist_out, const DecodeExtra& ex, hipStream_t st) {
   const bool aligned = ((reinterpret_cast<uintptr_t>(H) & 15) == 0);
   if (delta && P == 2 && Mr == D && (D == 64 || D == 32) && aligned) {
-    const size_t lds = sizeof(float) * std::max((size_t)(R + 1) * D, (size_t)D * (D + 4));
-    if (D == 64)
-      hipLaunchKernelGGL((decode_dd_kernel<64>), dim3(B), dim3(64), lds, st, H, y, xg, spins, R, M, ltrue, lhat, biterr,
-                         dist_out, ex);
-    else
-      hipLaunchKernelGGL((decode_dd_kernel<32>), dim3(B), dim3(64), lds, st, H, y, xg, spins, R, M, ltrue, lhat, biterr,
-                         dist_out, ex);
+    const bool lift = front_end_form() == 2;   // the complex-lift form: H is the real lift of a complex channel
+    const size_t lds = sizeof(float) * std::max((size_t)(R + 1) * D, (size_t)D * ((lift ? D / 2 : D) + 4));
+#define ARP_DD(DV, LV)                                                                                                \
+  hipLaunchKernelGGL((decode_dd_kernel<DV, LV>), dim3(B), dim3(64), lds, st, H, y, xg, spins, R, M, ltrue, lhat,     \
+                     biterr, dist_out, ex)
+    if (D == 64 && lift) ARP_DD(64, true);
+    else if (D == 64) ARP_DD(64, false);
+    else if (lift) ARP_DD(32, true);
+    else ARP_DD(32, false);
+#undef ARP_DD
     return hipGetLastError();
   }
   if (D <= 64 && Mr <= 64) {

@@ -34,6 +34,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 static int g_front_end_form = 0;
 void set_front_end_form(int form) { g_front_end_form = form; }
+int front_end_form() { return g_front_end_form; }
 
 __device__ __forceinline__ uint16_t bf16_of(float f) {
   const uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -415,7 +416,8 @@ hipError_t launch_front_end(const float* H, const float* y, const float* N0v, fl
   // (this kernel rounds G (Q / scale), equal to (G / scale) Q only for Q = 2^k) take the register-sweep kernels
   int qe = 0;
   const bool q_pow2 = Q > 0.f && std::frexp(Q, &qe) == 0.5f;
-  if (g_front_end_form == 0 && D <= 64 && Mr <= 64 && !o.J && !o.Jf && q_pow2) {
+  if (g_front_end_form == 2 && front_end_cpx(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st)) return hipGetLastError();
+  if (g_front_end_form != 1 && D <= 64 && Mr <= 64 && !o.J && !o.Jf && q_pow2) {
     if (D <= 16) launch_mfma<16>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);
     else if (D <= 32) launch_mfma<32>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);
     else if (D <= 48) launch_mfma<48>(H, y, N0v, N0s, B, Mr, D, M, K, Q, o, st);

@@ -70,6 +70,11 @@ hipError_t launch_front_end(const float* H, const float* y, const float* N0v, fl
 hipError_t launch_front_end_sweep(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D,
                                   int M, int K, float Q, const FrontOut& o, hipStream_t st);
 void set_front_end_form(int form);
+int front_end_form();  // 2: the frames' H are real lifts of complex channels (front end and decodes read half of H)
+// The complex-lift front end (front_end_cpx.hip) for H = [[Re, -Im], [Im, Re]], D = Mr in {32, 64}; false (nothing
+// launched) for other shapes, the expanded Chord inputs, a Q that is not a power of two, or a misaligned H.
+bool front_end_cpx(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M, int K,
+                   float Q, const FrontOut& o, hipStream_t st);
 // The MDI sub-problem build (Gi, hi around a given decision xg for the frames of a list) on the matrix-core front end;
 // false (nothing launched) for D or Mr > 64 or a Q that is not a power of two.
 bool mdi_build_mfma(const float* H, const float* y, const int8_t* xg, const int32_t* list, int n, int Mr, int D,

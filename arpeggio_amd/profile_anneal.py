@@ -3,10 +3,11 @@
 
     python3 -m arpeggio_amd.profile_anneal run [--start rand|learned] [--passes 64] [--frames 65536] [--copies 16]
                                                [--launches 3] [--snr 23.68] [--pipeline plain|adaptive]
-                                               [--stage1 4,24] [--c 6]
+                                               [--stage1 4,24] [--c 6] [--front-end-form 2]
         draws the 32x32 16-QAM frames once, then runs the full step `launches` times; every kernel of the path appears
         `launches` times in a trace. plain: front end [+ MLP] + anneal + decode. adaptive: the two-stage detector
         (front end [+ MLP], stage-1 anneal and decode, certificate, stage-2 anneal and decode of the open frames).
+        --front-end-form: 2 (default, as the bench) the complex-lift front end, 0 the general matrix-core form.
     python3 -m arpeggio_amd.profile_anneal copy
         three device copies of 256 MiB: the known-size reference for scaling FETCH_SIZE / WRITE_SIZE.
     python3 -m arpeggio_amd.profile_anneal traffic FETCH_DB WRITE_DB COPY_FETCH_DB COPY_WRITE_DB [--match arpeggio]
@@ -23,7 +24,8 @@ COPY_BYTES = 256 << 20
 
 def run(a):
     import torch
-    from . import frames
+    from . import _lib, frames
+    _lib.check(_lib.load().arp_set_front_end_form(a.front_end_form), 'front end form')
     from .detector import DetectPipeline
     model = None
     if a.start == 'learned':
@@ -95,6 +97,7 @@ def main():
     r.add_argument('--pipeline', default='plain', choices=('plain', 'adaptive'))
     r.add_argument('--stage1', default='4,24', help='adaptive stage 1: anneals,passes')
     r.add_argument('--c', type=float, default=6.0, help='adaptive certificate constant')
+    r.add_argument('--front-end-form', type=int, default=2, choices=(0, 1, 2))
     sub.add_parser('copy')
     t = sub.add_parser('traffic')
     t.add_argument('dbs', nargs=4)

@@ -75,6 +75,8 @@ def parse():
     ap.add_argument('--no-mdi', action='store_true', help='skip the MDI wrapper against the reference fixtures')
     ap.add_argument('--mdi-batches', type=int, default=60, help='32-user fixture batches (1,000 frames each)')
     ap.add_argument('--mdi-large', type=int, default=16384, help='frames of the MDI throughput batch')
+    ap.add_argument('--front-end-form', type=int, default=2, choices=(0, 1, 2),
+                    help='2: the complex-lift front end (the frames are real lifts of complex channels), 0: general')
     return ap.parse_args()
 
 
@@ -539,7 +541,10 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
-    from arpeggio_amd import frames
+    from arpeggio_amd import _lib, frames
+    # every frame here is the real lift of a complex channel (frames.draw, as mimo.complex_to_real builds it), so the
+    # front end may take its complex-lift form: Hermitian Gram and complex sweep from half of H
+    _lib.check(_lib.load().arp_set_front_end_form(args.front_end_form), 'front end form')
     from arpeggio_amd.stats import boot_ratio
     B, R, L = args.frames, args.copies, args.passes
     if args.snr is None:
@@ -626,7 +631,9 @@ def main():
                                   'plain budget on open frames; %s step timed' % (B, R, L, args.stage1, timed),
                          frames=B, users=NT, rx=NT, M=M, copies=R, plain_passes=L, stage1=args.stage1,
                          policy='arpeggio', snr_db=snr, target_ber=args.target_ber, calibration=calib,
-                         T0=0.1, T1=0.1, q=0.0)
+                         T0=0.1, T1=0.1, q=0.0,
+                         front_end='complex lift (Hermitian Gram, half of H read)' if args.front_end_form == 2
+                         else ('register sweep' if args.front_end_form == 1 else 'general matrix-core sweep'))
     if not args.no_cpu:
         log('cpu baseline (3 segments)')
         out['cpu_baseline'] = cpu_baseline(b, plain, args)

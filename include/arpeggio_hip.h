@@ -45,8 +45,12 @@ int arp_set_anneal_waves(int waves);
 int arp_set_anneals_per_wave(int apw);
 
 /* Front-end schedule for D, Mr <= 64 (no counterpart in the reference, which has one float front end): 0 (default)
- * the matrix-core block sweep, 1 the register sweep. Both compute the same quantities; the results agree to float32
- * rounding. Process-wide. */
+ * the matrix-core block sweep, 1 the register sweep, 2 the complex-lift form for D = Mr in {32, 64} (other shapes as
+ * 0): H must be the real lift of a complex channel, H = [[Re, -Im], [Im, Re]] (mimo.complex_to_real,
+ * arpeggio/mimo/mimo.py:55-58), of which only the first D/2 columns are read; the Gram A + iB is Hermitian and the
+ * inverse is swept in complex form, with about half the matrix-core work. With form 2 the two-copy decodes (D = Mr in
+ * {32, 64}) also read only those columns (same distances bit for bit). All forms compute the same quantities; the
+ * results agree to float32 rounding. Process-wide. */
 int arp_set_front_end_form(int form);
 
 /* Philox4x32-10 known-answer kernel: in (n, 6) uint32 rows (counter[4], key[2]) -> out (n, 4).

@@ -32,12 +32,14 @@ def fe_form():
     L.arp_set_front_end_form(0)
 
 
-@pytest.mark.parametrize('form', [0, 1])
+@pytest.mark.parametrize('form', [0, 1, 2])
 @pytest.mark.parametrize('nt,M,snr,K', [(16, 16, 16.0, 1), (32, 16, 20.0, 1), (16, 4, 8.0, 1), (64, 16, 22.0, 1),
-                                        (16, 16, 14.0, 2), (8, 16, 12.0, 3), (31, 16, 20.0, 1), (24, 4, 10.0, 1)])
+                                        (16, 16, 14.0, 2), (8, 16, 12.0, 3), (31, 16, 20.0, 1), (24, 4, 10.0, 1),
+                                        (32, 4, 10.0, 2)])
 def test_front_end_vs_oracle(orc, fe_form, nt, M, snr, K, form):
-    """Both front-end forms (0: matrix-core block sweep for D, Mr <= 64; 1: register sweep) against the oracle; the
-    int16 Gram equals the int32 one; the expanded Chord inputs (register-sweep kernels) expand the same integers."""
+    """The front-end forms (0: matrix-core block sweep for D, Mr <= 64; 1: register sweep; 2: the complex-lift form
+    for 16 and 32 users, the others as 0) against the oracle; the int16 Gram equals the int32 one; the expanded Chord
+    inputs (register-sweep kernels) expand the same integers."""
     from arpeggio_amd.detector import front_end
     say('front_end', nt, M, snr, K, 'form', form)
     fe_form(form)
@@ -74,6 +76,53 @@ def test_front_end_vs_oracle(orc, fe_form, nt, M, snr, K, form):
         torch.cuda.synchronize()
         J, hN = orc.expand(oj['Gi'].cpu().numpy().astype(np.int64), oj['hi'].cpu().numpy(), K)
         assert (oj['J'].cpu().numpy() == J).all() and (oj['hN'].cpu().numpy() == hN).all()
+
+
+@pytest.mark.parametrize('nt,M,snr', [(32, 16, 23.7), (16, 16, 16.0), (32, 4, 10.0)])
+def test_complex_lift_form_against_general(fe_form, nt, M, snr):
+    """Form 2 (Hermitian Gram and complex sweep from half of H) against form 0 on the same real-lift frames, learned
+    features included: decisions equal but for float near-ties, integers within one unit."""
+    from arpeggio_amd.detector import front_end
+    from arpeggio_amd.learned import feat_ld, features
+    say('complex lift vs general', nt, M, snr)
+    b = frames(nt, M, snr, 4096, seed=77)
+    out = []
+    for form in (0, 2):
+        fe_form(form)
+        o = front_end(b['H'], b['y'], b['N0'], M, 1, want=('xg', 'Gi', 'hi', 'scale', 'z', 'mu'))
+        o['feat'] = features(b['H'], b['y'], b['N0'], M)['feat']
+        torch.cuda.synchronize()
+        out.append({k: v.cpu().float().numpy() if v.dtype == torch.bfloat16 else v.cpu().numpy() for k, v in o.items()})
+    a, c = out
+    assert a['feat'].shape[1] == feat_ld(2 * nt)
+    assert (a['xg'] == c['xg']).mean() > 0.9999
+    same = (a['xg'] == c['xg']).all(1)
+    assert np.abs(a['Gi'][same] - c['Gi'][same]).max() <= 1 and (a['Gi'][same] == c['Gi'][same]).mean() > 0.999
+    assert np.abs(a['hi'][same] - c['hi'][same]).max() <= 2
+    np.testing.assert_allclose(c['z'], a['z'], rtol=1e-3, atol=1e-3)
+    np.testing.assert_allclose(c['mu'], a['mu'], rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(c['scale'][same], a['scale'][same], rtol=1e-4)
+    np.testing.assert_allclose(c['feat'][same], a['feat'][same], rtol=2e-2, atol=2e-2)
+
+
+@pytest.mark.parametrize('nt,M', [(32, 16), (16, 16), (32, 4)])
+def test_decode_lift_form_equals_general(fe_form, nt, M):
+    """With form 2 the two-copy decode reads only the first D/2 columns of the real-lift H and rebuilds each row: the
+    distances, decisions and bit errors equal the full-H decode's bit for bit."""
+    from arpeggio_amd.detector import decode_select, front_end
+    say('decode lift', nt, M)
+    b = frames(nt, M, 14.0, 2048, seed=5)
+    o = front_end(b['H'], b['y'], b['N0'], M, 1, want=('xg',))
+    g = torch.Generator(device='cuda').manual_seed(3)
+    spins = (torch.randint(0, 2, (2048, 9, 4 * nt), generator=g, device='cuda') * 2 - 1).to(torch.int8)
+    res = []
+    for form in (0, 2):
+        fe_form(form)
+        lhat, be, dist = decode_select(b['H'], b['y'], o['xg'], spins, M, P=2, delta=True, ltrue=b['l'],
+                                       want_dist=True)
+        torch.cuda.synchronize()
+        res.append((lhat.clone(), be.clone(), dist.clone()))
+    assert all(torch.equal(x, z) for x, z in zip(*res))
 
 
 def test_front_end_golden(golden):

@@ -17,12 +17,23 @@ def near_half(x, tol):
     return np.abs(np.abs(x - np.floor(x)) - 0.5) < tol
 
 
+@pytest.fixture
+def fe_form():
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    yield lambda f: _lib.check(L.arp_set_front_end_form(f), 'set_front_end_form')
+    L.arp_set_front_end_form(0)
+
+
+@pytest.mark.parametrize('form', [0, 2])
 @pytest.mark.parametrize('nt,M', [(16, 16), (32, 16), (64, 16), (32, 4)])
-def test_front_end_per_frame_n0(orc, nt, M):
-    """Per-frame N0 (a (B,) tensor of distinct values, 8-24 dB) against the oracle run frame by frame."""
+def test_front_end_per_frame_n0(orc, fe_form, nt, M, form):
+    """Per-frame N0 (a (B,) tensor of distinct values, 8-24 dB) against the oracle run frame by frame (form 2: the
+    complex-lift front end where it applies)."""
     from arpeggio_amd import frames
     from arpeggio_amd.detector import front_end
-    say('front end per-frame N0', nt, M)
+    say('front end per-frame N0', nt, M, 'form', form)
+    fe_form(form)
     B = 24
     snrs = np.linspace(8.0, 24.0, B)
     b = frames.draw_mixed_snr(91, snrs, 0, nt, M)
