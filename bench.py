@@ -282,21 +282,24 @@ def profile_record(name, key):
     return (d.get('launches') or d.get('kernels') or {}).get(key)
 
 
-def roofline_front_end(t_ms, B, learned):
+def roofline_front_end(t_ms, B, learned, form=2):
     """The front end: LMMSE + Gram on the matrix cores. Algorithmic work per frame (the Gram, H^T H, and H^T y):
-    2 (2Nr)(2Nt)^2 + 2 (2Nr)(2Nt) FLOP; the bound is the FP32 MFMA peak. traffic: the measured HBM bytes per launch
-    (FETCH_SIZE + WRITE_SIZE, scaled by a device copy's reading); counters: the matrix-core and vector busy fractions
-    of the same kernel (SQ counters), which also count the inversion's MFMAs."""
+    2 (2Nr)(2Nt)^2 + 2 (2Nr)(2Nt) FLOP, the real-lift count, for either form; the bound is the FP32 MFMA peak.
+    Algorithmic HBM bytes: H (half of it for the complex-lift form 2, which reads [Re; Im] only), y, and the outputs.
+    traffic: the measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a device copy's reading);
+    counters: the matrix-core and vector busy fractions of the same kernel (SQ counters), which also count the
+    inversion's MFMAs."""
     D = Mr = 2 * NT
     flop = B * (2 * Mr * D * D + 2 * Mr * D)
     achieved = flop / (t_ms * 1e-3) / 1e12
-    hbm = B * (4 * Mr * D + 4 * Mr + 2 * D * D + 4 * D + D + 4 + (2 * feat_len() if learned else 0))
-    key = ('adaptive' if learned else 'plain') + '/front_end_mfma'
+    hcols = D // 2 if form == 2 else D
+    hbm = B * (4 * Mr * hcols + 4 * Mr + 2 * D * D + 4 * D + D + 4 + (2 * feat_len() if learned else 0))
+    key = ('adaptive' if learned else 'plain') + '/front_end'
     tr = profile_record('detect_step_traffic.json', key)
     ct = profile_record('detect_step_counters.json', key)
     return dict(bound='mfma', achieved=achieved, peak=MFMA_FP32_TFLOPS, unit='TFLOP/s',
                 frac=achieved / MFMA_FP32_TFLOPS, traffic=tr['hbm_bytes_per_launch'] if tr else None,
-                kernel='front_end_mfma_kernel<64>', launch_ms=t_ms, algorithmic_flop=flop, hbm_algorithmic_bytes=hbm,
+                kernel='front_end_cpx_kernel<64>' if form == 2 else 'front_end_mfma_kernel<64>', launch_ms=t_ms, algorithmic_flop=flop, hbm_algorithmic_bytes=hbm,
                 hbm=dict(achieved_GBs=hbm / (t_ms * 1e-3) / 1e9, peak_GBs=HBM_PEAK_GBS,
                          frac=hbm / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
                 counters=dict(mfma_busy=ct['mfma_busy'], vector_busy=ct['vector_busy'],
@@ -612,8 +615,10 @@ def main():
         out['speedup_vs_plain'] = tl['value'] / tp['value']
         st = out['learned']['stages']
         timed = 'learned (adaptive)'
+        fe_roof = roofline_front_end(st['front_end'], B, True, args.front_end_form)
+        out['learned']['front_end_roofline'] = fe_roof
         if st['front_end'] >= st['anneal_1']:
-            out['roofline'] = roofline_front_end(st['front_end'], B, True)
+            out['roofline'] = fe_roof
         else:
             fl = float(lp.st1.double()[..., 0].mean())
             out['roofline'] = roofline_anneal(st['anneal_1'], B, lp.R1, lp.L1, fl, True,
