@@ -411,7 +411,18 @@ __global__ __launch_bounds__(128) void decode_w128_kernel(const float* __restric
   const float* Hb = H + (size_t)b * Mr * D;
   const bool row = t < Mr;
   float hr[DP];
-  if (vec4) {
+  if (vec4 == 2) {
+    // the real lift of a complex channel (D = Mr = 128): only the first 64 columns are read, and row t's last 64
+    // entries are row t + 64's first (negated) for t < 64, row t - 64's first for t >= 64: the stored values
+    const int t2 = t < 64 ? t + 64 : t - 64;
+    const float sg = t < 64 ? -1.f : 1.f;
+#pragma unroll
+    for (int q = 0; q < DP / 4; ++q) {
+      const floatx4 v = *reinterpret_cast<const floatx4*>(Hb + (size_t)(q < 16 ? t : t2) * D + 4 * (q & 15));
+#pragma unroll
+      for (int e = 0; e < 4; ++e) hr[4 * q + e] = q < 16 ? v[e] : sg * v[e];
+    }
+  } else if (vec4) {
 #pragma unroll
     for (int q = 0; q < DP / 4; ++q) {
       floatx4 v = {0.f, 0.f, 0.f, 0.f};
@@ -541,7 +552,9 @@ hipError_t launch_decode_ex(const float* H, const float* y, const int8_t* xg, co
     if (l128 > 65536)
       hipFuncSetAttribute((const void*)decode_w128_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l128);
     const int vec4 = (D % 4 == 0) && ((reinterpret_cast<uintptr_t>(H) & 15) == 0);
-    hipLaunchKernelGGL(decode_w128_kernel, dim3(B), dim3(128), l128, st, H, y, xg, spins, R, P, D, Mr, M, delta, vec4,
+    // vec4 = 2: the complex-lift form reads half of H
+    const int lift = vec4 && D == 128 && Mr == 128 && front_end_form() == 2 ? 2 : vec4;
+    hipLaunchKernelGGL(decode_w128_kernel, dim3(B), dim3(128), l128, st, H, y, xg, spins, R, P, D, Mr, M, delta, lift,
                        ltrue, lhat, biterr, dist_out, ex);
     return hipGetLastError();
   }

@@ -1,4 +1,4 @@
-// Front end for the real lift of a complex channel on the matrix cores: 16 or 32 users, as many receive antennas.
+// Front end for the real lift of a complex channel on the matrix cores: 16, 32 or 64 users, as many receive antennas.
 //
 // The real lift H = [[Re, -Im], [Im, Re]] (mimo.complex_to_real, arpeggio/mimo/mimo.py:55-58) has a Gram of the same
 // form, G = [[A, -B], [B, A]] with A = Re^T Re + Im^T Im symmetric and B = Re^T Im - Im^T Re antisymmetric: the real
@@ -385,15 +385,371 @@ __global__ __launch_bounds__(64) void front_end_cpx_kernel(const float* __restri
   }
 }
 
+
+// 64 users (D = Mr = 128: n = 64 complex coordinates, 4 x 4 tiles of A and of B): the same algorithm with one lane per
+// complex coordinate in the per-coordinate phases (lane j owns both real coordinates j and n + j), and the Gram's
+// k-steps taken in pairs (s, s + 16): rows r = 4 s + grp (top) and r + 64 (bottom) of C1, whose D1 rows are each other
+// (D1[r] = C1[r + 64], D1[r + 64] = -C1[r]). GIVEN: the MDI sub-problem build around given decisions (mimo.mdi's dDI
+// step, arpeggio/mimo/mimo.py:570-590): x_g read from xg_in, no inversion, workgroup i builds frame list[i].
+template <bool GIVEN>
+__global__ __launch_bounds__(64) void front_end_cpx128_kernel(const float* __restrict__ H, const float* __restrict__ y,
+                                                              const float* __restrict__ N0v, float N0s, int M, int K,
+                                                              float Q, FrontOut o, const int8_t* __restrict__ xg_in,
+                                                              const int32_t* __restrict__ list) {
+  constexpr int DP = 128, n = 64, nC = 4, nU = 10, Mr = 128, SH = 16, TS = 260;
+  __shared__ __align__(16) float At[nU * TS];
+  __shared__ __align__(16) float Bt[nU * TS];
+  __shared__ __align__(16) float pan[n * 4];
+  __shared__ __align__(16) float cz[2 * n];
+  __shared__ __align__(16) float xs[4 * n];
+  const int b = GIVEN ? list[blockIdx.x] : (int)blockIdx.x, lane = threadIdx.x, col = lane & 15, grp = lane >> 4;
+  const float* Hb = H + (size_t)b * Mr * DP;
+  const float* yb = y + (size_t)b * Mr;
+  const float ylo = yb[lane], yhi = yb[n + lane];
+
+  // 1. Gram and c over the row pairs (4 s + grp, 4 s + grp + 64)
+  floatx4 TA[nU], TB[nU];
+  float cpr[nC], cpi[nC];
+#pragma unroll
+  for (int u = 0; u < nU; ++u) {
+    TA[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+    TB[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+  }
+#pragma unroll
+  for (int I = 0; I < nC; ++I) cpr[I] = cpi[I] = 0.f;
+#pragma unroll
+  for (int s = 0; s < SH; ++s) {
+    if (s % 4 == 0) __builtin_amdgcn_sched_barrier(0);  // H loads in groups of four row pairs: bounded registers
+    const int ra = 4 * s + grp;
+    const floatx4 a = *reinterpret_cast<const floatx4*>(Hb + (size_t)ra * DP + nC * col);
+    const floatx4 c = *reinterpret_cast<const floatx4*>(Hb + (size_t)(ra + n) * DP + nC * col);
+    const float ya = __shfl(ylo, ra), yb2 = __shfl(yhi, ra);  // y[ra], y[ra + 64]
+    // c_r = C1^T y; c_i = C1^T [y_bot; -y_top]: row ra takes y[ra + 64], row ra + 64 takes -y[ra]
+#pragma unroll
+    for (int I = 0; I < nC; ++I) {
+      cpr[I] = __builtin_fmaf(c[I], yb2, __builtin_fmaf(a[I], ya, cpr[I]));
+      cpi[I] = __builtin_fmaf(c[I], -ya, __builtin_fmaf(a[I], yb2, cpi[I]));
+    }
+#pragma unroll
+    for (int I = 0; I < nC; ++I)
+#pragma unroll
+      for (int J = I; J < nC; ++J) {
+        const int u = utile<nC>(I, J);
+        TA[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[I], a[J], TA[u], 0, 0, 0);
+        TA[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(c[I], c[J], TA[u], 0, 0, 0);
+        TB[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[I], c[J], TB[u], 0, 0, 0);
+        TB[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(c[I], -a[J], TB[u], 0, 0, 0);
+      }
+  }
+  float cr[nC], ci[nC];
+#pragma unroll
+  for (int I = 0; I < nC; ++I) {
+    float v = cpr[I], w = cpi[I];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    w += __shfl_xor(w, 16);
+    w += __shfl_xor(w, 32);
+    cr[I] = v;
+    ci[I] = w;
+    if (grp == 0) {
+      cz[16 * I + col] = v;
+      cz[n + 16 * I + col] = w;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < nU; ++u) {
+    *reinterpret_cast<floatx4*>(At + u * TS + 16 * col + 4 * grp) = TA[u];
+    *reinterpret_cast<floatx4*>(Bt + u * TS + 16 * col + 4 * grp) = TB[u];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int I = 0; I < nC; ++I) {
+    const float* bt = Bt + utile<nC>(I, I) * TS + col;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) TB[utile<nC>(I, I)][v] = (TB[utile<nC>(I, I)][v] - bt[16 * (4 * grp + v)]) * 0.5f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int I = 0; I < nC; ++I)
+    *reinterpret_cast<floatx4*>(Bt + utile<nC>(I, I) * TS + 16 * col + 4 * grp) = TB[utile<nC>(I, I)];
+  const float el2 = M == 4 ? 1.f : 5.f;
+  const float n0 = GIVEN ? 0.f : (N0v ? N0v[b] : N0s);
+  const float sigma = (n0 / 2.f) / el2;
+#pragma unroll
+  for (int I = 0; I < nC; ++I)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (4 * grp + v == col) TA[utile<nC>(I, I)][v] += sigma;
+
+  // 2. Complex block sweep (as front_end_cpx_kernel); the eight pivot pairs of a tile column run as a loop, so that
+  // the 32 blocks do not all unroll (in-tile row q0 at run time: the pair's registers v0, v0 + 1 by selects)
+  const int kk = grp & 1;
+#pragma unroll
+  for (int J0 = 0; J0 < (GIVEN ? 0 : nC); ++J0)
+#pragma unroll 1
+  for (int bq = 0; bq < 8; ++bq) {
+    const int q0 = 2 * bq, g0 = q0 >> 2, v0 = q0 & 3, p0 = 16 * J0 + q0;
+    if (grp == g0) {
+#pragma unroll
+      for (int I = J0; I < nC; ++I) {
+        const floatx4 ta = TA[utile<nC>(J0, I)], tb = TB[utile<nC>(J0, I)];
+        *reinterpret_cast<floatx4*>(pan + (16 * I + col) * 4) =
+            floatx4{v0 ? ta[2] : ta[0], v0 ? ta[3] : ta[1], v0 ? tb[2] : tb[0], v0 ? tb[3] : tb[1]};
+      }
+    }
+    if ((col >> 1) == (q0 >> 1)) {
+      const int e = col - q0;
+#pragma unroll
+      for (int I = 0; I < J0; ++I)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          pan[(16 * I + 4 * grp + v) * 4 + e] = TA[utile<nC>(I, J0)][v];
+          pan[(16 * I + 4 * grp + v) * 4 + 2 + e] = -TB[utile<nC>(I, J0)][v];
+        }
+    }
+    __syncthreads();
+    const floatx4 r0 = *reinterpret_cast<const floatx4*>(pan + p0 * 4);
+    const floatx4 r1 = *reinterpret_cast<const floatx4*>(pan + (p0 + 1) * 4);
+    const float a = r0[0], d = r1[1], al = r1[0], be = r1[2];
+    const float rdet = __builtin_amdgcn_rcpf(__builtin_fmaf(a, d, -__builtin_fmaf(al, al, be * be)));
+    const float m00 = d * rdet, m11 = a * rdet, m01r = -al * rdet, m01i = -be * rdet;
+    const float Mr0 = kk ? m01r : m00, Mi0 = kk ? m01i : 0.f, Mr1 = kk ? m11 : m01r, Mi1 = kk ? 0.f : -m01i;
+    const float c0r = lane_val(cr[J0], q0), c1r = lane_val(cr[J0], q0 + 1);
+    const float c0i = lane_val(ci[J0], q0), c1i = lane_val(ci[J0], q0 + 1);
+    const float q0r = __builtin_fmaf(m00, c0r, __builtin_fmaf(m01r, c1r, -m01i * c1i));
+    const float q0i = __builtin_fmaf(m00, c0i, __builtin_fmaf(m01r, c1i, m01i * c1r));
+    const float q1r = __builtin_fmaf(m01r, c0r, __builtin_fmaf(m01i, c0i, m11 * c1r));
+    const float q1i = __builtin_fmaf(m01r, c0i, __builtin_fmaf(-m01i, c0r, m11 * c1i));
+    float nuA[nC], nuB[nC], vt[nC];
+#pragma unroll
+    for (int I = 0; I < nC; ++I) {
+      const floatx4 x = *reinterpret_cast<const floatx4*>(pan + (16 * I + col) * 4);
+      const float vr0 = x[0] - (I == J0 && col == q0 ? 1.f : 0.f), vr1 = x[1] - (I == J0 && col == q0 + 1 ? 1.f : 0.f);
+      const float vi0 = -x[2], vi1 = -x[3];
+      const float ur = __builtin_fmaf(vr0, Mr0, __builtin_fmaf(-vi0, Mi0, __builtin_fmaf(vr1, Mr1, -vi1 * Mi1)));
+      const float ui = __builtin_fmaf(vr0, Mi0, __builtin_fmaf(vi0, Mr0, __builtin_fmaf(vr1, Mi1, vi1 * Mr1)));
+      nuA[I] = grp < 2 ? -ur : -ui;
+      nuB[I] = grp < 2 ? -ui : ur;
+      cr[I] -= __builtin_fmaf(vr0, q0r, __builtin_fmaf(-vi0, q0i, __builtin_fmaf(vr1, q1r, -vi1 * q1i)));
+      ci[I] -= __builtin_fmaf(vr0, q0i, __builtin_fmaf(vi0, q0r, __builtin_fmaf(vr1, q1i, vi1 * q1r)));
+      const float w = pan[(16 * I + col) * 4 + grp];
+      vt[I] = grp < 2 ? w - (I == J0 && col == q0 + grp ? 1.f : 0.f) : -w;
+    }
+#pragma unroll
+    for (int I = 0; I < nC; ++I)
+#pragma unroll
+      for (int J = I; J < nC; ++J) {
+        TA[utile<nC>(I, J)] = __builtin_amdgcn_mfma_f32_16x16x4f32(nuA[I], vt[J], TA[utile<nC>(I, J)], 0, 0, 0);
+        TB[utile<nC>(I, J)] = __builtin_amdgcn_mfma_f32_16x16x4f32(nuB[I], vt[J], TB[utile<nC>(I, J)], 0, 0, 0);
+      }
+  }
+
+  // 3. Lane j = complex coordinate (position p = 16 (j % 4) + j / 4): both real coordinates j and n + j.
+  const int j = lane, Ii = j % nC, mi = j / nC, pp = 16 * Ii + mi;
+  const int src = mi + 16 * (mi >> 2);
+  float zr = 0.f, zi = 0.f, dgi = 0.f;
+#pragma unroll
+  for (int I = 0; I < nC; ++I) {
+    const float a = __shfl(cr[I], mi), c = __shfl(ci[I], mi);
+    const float td = __shfl(pick4c(TA[utile<nC>(I, I)], col & 3), src);
+    if (Ii == I) {
+      zr = a;
+      zi = c;
+      dgi = 2.f - td;
+    }
+  }
+  const float gii = At[(Ii * nC - Ii * (Ii - 1) / 2) * TS + 17 * mi];
+  const float c0r = cz[pp], c0i = cz[n + pp];
+  const float lmax = M == 4 ? 1.f : 3.f;
+  float lr, li, vr = 0.f, vi = 0.f, mu = 1.f;
+  if (GIVEN) {
+    lr = (float)xg_in[(size_t)b * DP + j];
+    li = (float)xg_in[(size_t)b * DP + n + j];
+  } else {
+    mu = 1.f - sigma * dgi;
+    vr = zr / mu;
+    vi = zi / mu;
+    lr = fminf(fmaxf(2.f * ceilf(vr * 0.5f) - 1.f, -lmax), lmax);
+    li = fminf(fmaxf(2.f * ceilf(vi * 0.5f) - 1.f, -lmax), lmax);
+    if (vr != vr) lr = -lmax;
+    if (vi != vi) li = -lmax;
+    if (o.z) {
+      o.z[(size_t)b * DP + j] = zr;
+      o.z[(size_t)b * DP + n + j] = zi;
+    }
+    if (o.mu) {
+      o.mu[(size_t)b * DP + j] = mu;
+      o.mu[(size_t)b * DP + n + j] = mu;
+    }
+    if (o.xg) {
+      o.xg[(size_t)b * DP + j] = (int8_t)lr;
+      o.xg[(size_t)b * DP + n + j] = (int8_t)li;
+    }
+    if (o.feat) {
+      float gs = gii;
+      for (int off = 32; off; off >>= 1) gs += __shfl_xor(gs, off);
+      const float gbar = gs / (float)n;   // each A_jj is G_ii for two real coordinates
+      uint16_t* f = o.feat + (size_t)b * o.feat_ld;
+      f[j] = bf16_bits(c0r / gbar);
+      f[n + j] = bf16_bits(c0i / gbar);
+      f[DP + j] = bf16_bits(gii / gbar - 1.f);
+      f[DP + n + j] = bf16_bits(gii / gbar - 1.f);
+      f[2 * DP + j] = bf16_bits(lr / lmax);
+      f[2 * DP + n + j] = bf16_bits(li / lmax);
+      f[3 * DP + j] = bf16_bits(vr - lr);
+      f[3 * DP + n + j] = bf16_bits(vi - li);
+      for (int x = 4 * DP + lane; x < o.feat_ld; x += 64) f[x] = bf16_bits(x == 4 * DP ? sigma / gbar : 0.f);
+    }
+  }
+
+  // 4. h = 2 (c - G x_g) for both parts of row j: A_j x_r - B_j x_i and B_j x_r + A_j x_i
+  xs[pp] = lr;
+  xs[n + pp] = li;
+  __syncthreads();
+  float g0 = 0.f, g1 = 0.f, gm = 0.f;
+#pragma unroll
+  for (int J = 0; J < nC; ++J) {
+    const bool up = J >= Ii;
+    const int tl = up ? Ii * nC - Ii * (Ii - 1) / 2 + (J - Ii) : J * nC - J * (J - 1) / 2 + (Ii - J);
+    const float* ga = At + tl * TS + (up ? mi : 16 * mi);
+    const float* gb = Bt + tl * TS + (up ? mi : 16 * mi);
+    const int st = up ? 16 : 1;
+    const float sb = up ? 1.f : -1.f;
+#pragma unroll
+    for (int c4 = 0; c4 < 16; c4 += 4) {
+      const floatx4 ur = *reinterpret_cast<const floatx4*>(xs + 16 * J + c4);
+      const floatx4 ui = *reinterpret_cast<const floatx4*>(xs + n + 16 * J + c4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float ga_ = ga[st * (c4 + e)], gb_ = sb * gb[st * (c4 + e)];
+        g0 = __builtin_fmaf(-gb_, ui[e], __builtin_fmaf(ga_, ur[e], g0));
+        g1 = __builtin_fmaf(ga_, ui[e], __builtin_fmaf(gb_, ur[e], g1));
+        gm = fmaxf(gm, fmaxf(fabsf(ga_), fabsf(gb_)));
+      }
+    }
+  }
+  const float h0 = 2.f * (c0r - g0), h1 = 2.f * (c0i - g1);
+  float mx = fmaxf(gm, fmaxf(fabsf(h0), fabsf(h1)));
+  for (int off = 32; off; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  const float scale = fmaxf(mx, 1e-12f);
+  const float inv = 1.f / scale;
+  if (lane == 0 && o.scale) o.scale[b] = scale;
+  {
+    const int P = 2 * K, N = P * DP;
+    const float hs0 = h0 * inv, hs1 = h1 * inv;
+    const int32_t q0 = (int32_t)rintf(hs0 * Q), q1 = (int32_t)rintf(hs1 * Q);
+    if (o.hi) {
+      o.hi[(size_t)b * DP + j] = q0;
+      o.hi[(size_t)b * DP + n + j] = q1;
+    }
+    if (o.hf) {
+      o.hf[(size_t)b * DP + j] = hs0;
+      o.hf[(size_t)b * DP + n + j] = hs1;
+    }
+    for (int p = 0; p < P; ++p) {
+      if (o.hN) {
+        o.hN[(size_t)b * N + p * DP + j] = q0;
+        o.hN[(size_t)b * N + p * DP + n + j] = q1;
+      }
+      if (o.hNf) {
+        o.hNf[(size_t)b * N + p * DP + j] = hs0;
+        o.hNf[(size_t)b * N + p * DP + n + j] = hs1;
+      }
+    }
+  }
+
+  // 5. G / scale row-major, rows (part, jr), jr = 4 (4 grp + k) + I, columns 4 col .. 4 col + 3 of each half
+  if (o.Gi || o.Gf || o.Gh) {
+    int32_t* Gib = o.Gi ? o.Gi + (size_t)b * DP * DP : nullptr;
+    int16_t* Ghb = o.Gh ? o.Gh + (size_t)b * DP * DP : nullptr;
+    float* Gfb = o.Gf ? o.Gf + (size_t)b * DP * DP : nullptr;
+    const float invQ = inv * Q;
+#pragma unroll
+    for (int I = 0; I < nC; ++I) {
+      floatx4 ba[nC], bb[nC];
+#pragma unroll
+      for (int J = 0; J < nC; ++J) {
+        if (J >= I) {
+          ba[J] = *reinterpret_cast<const floatx4*>(At + utile<nC>(I, J) * TS + 16 * col + 4 * grp);
+          bb[J] = *reinterpret_cast<const floatx4*>(Bt + utile<nC>(I, J) * TS + 16 * col + 4 * grp);
+        } else {
+          const float* a0 = At + utile<nC>(J, I) * TS + 64 * grp + col;
+          const float* b0 = Bt + utile<nC>(J, I) * TS + 64 * grp + col;
+          ba[J] = floatx4{a0[0], a0[16], a0[32], a0[48]};
+          bb[J] = floatx4{-b0[0], -b0[16], -b0[32], -b0[48]};
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int jr = nC * (4 * grp + k) + I;
+        const size_t r0 = (size_t)jr * DP + nC * col, r1 = (size_t)(n + jr) * DP + nC * col;
+        uint32_t qa[nC], qb[nC], qn[nC];
+#pragma unroll
+        for (int J = 0; J < nC; ++J) {
+          qa[J] = rne_bits(ba[J][k] * invQ);
+          qb[J] = rne_bits(bb[J][k] * invQ);
+          qn[J] = 0u - qb[J];
+        }
+        if (Ghb) {
+          const uint2 ua = make_uint2(__builtin_amdgcn_perm(qa[1], qa[0], 0x05040100u),
+                                      __builtin_amdgcn_perm(qa[3], qa[2], 0x05040100u));
+          *reinterpret_cast<uint2*>(Ghb + r0) = ua;
+          *reinterpret_cast<uint2*>(Ghb + r0 + n) = make_uint2(__builtin_amdgcn_perm(qn[1], qn[0], 0x05040100u),
+                                                               __builtin_amdgcn_perm(qn[3], qn[2], 0x05040100u));
+          *reinterpret_cast<uint2*>(Ghb + r1) = make_uint2(__builtin_amdgcn_perm(qb[1], qb[0], 0x05040100u),
+                                                           __builtin_amdgcn_perm(qb[3], qb[2], 0x05040100u));
+          *reinterpret_cast<uint2*>(Ghb + r1 + n) = ua;
+        }
+        if (Gib) {
+          *reinterpret_cast<int4*>(Gib + r0) = make_int4((int)qa[0], (int)qa[1], (int)qa[2], (int)qa[3]);
+          *reinterpret_cast<int4*>(Gib + r0 + n) = make_int4((int)qn[0], (int)qn[1], (int)qn[2], (int)qn[3]);
+          *reinterpret_cast<int4*>(Gib + r1) = make_int4((int)qb[0], (int)qb[1], (int)qb[2], (int)qb[3]);
+          *reinterpret_cast<int4*>(Gib + r1 + n) = make_int4((int)qa[0], (int)qa[1], (int)qa[2], (int)qa[3]);
+        }
+        if (Gfb) {
+#pragma unroll
+          for (int J = 0; J < nC; ++J) {
+            Gfb[r0 + J] = ba[J][k] * inv;
+            Gfb[r0 + n + J] = -(bb[J][k] * inv);
+            Gfb[r1 + J] = bb[J][k] * inv;
+            Gfb[r1 + n + J] = ba[J][k] * inv;
+          }
+        }
+      }
+    }
+  }
+}
+
 bool front_end_cpx(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M, int K,
                    float Q, const FrontOut& o, hipStream_t st) {
   int qe = 0;
-  if (Mr != D || (D != 32 && D != 64) || o.J || o.Jf || !(Q > 0.f) || std::frexp(Q, &qe) != 0.5f) return false;
-  if (D == 64 && reinterpret_cast<uintptr_t>(H) % 8) return false;  // float2 loads of a lane's two columns
-  if (D == 64)
+  if (Mr != D || (D != 32 && D != 64 && D != 128) || o.J || o.Jf || !(Q > 0.f) || std::frexp(Q, &qe) != 0.5f)
+    return false;
+  if (D == 64 && reinterpret_cast<uintptr_t>(H) % 8) return false;    // float2 loads of a lane's two columns
+  if (D == 128 && reinterpret_cast<uintptr_t>(H) % 16) return false;  // floatx4 loads of a lane's four columns
+  if (D == 128)
+    hipLaunchKernelGGL((front_end_cpx128_kernel<false>), dim3(B), dim3(64), 0, st, H, y, N0v, N0s, M, K, Q, o, nullptr,
+                       nullptr);
+  else if (D == 64)
     hipLaunchKernelGGL((front_end_cpx_kernel<64>), dim3(B), dim3(64), 0, st, H, y, N0v, N0s, M, K, Q, o);
   else
     hipLaunchKernelGGL((front_end_cpx_kernel<32>), dim3(B), dim3(64), 0, st, H, y, N0v, N0s, M, K, Q, o);
+  return true;
+}
+
+}  // namespace arp
+
+namespace arp {
+
+bool mdi_build_cpx128(const float* H, const float* y, const int8_t* xg, const int32_t* list, int n, float Q,
+                      int32_t* Gi, int32_t* hi, hipStream_t st) {
+  int qe = 0;
+  if (!(Q > 0.f) || std::frexp(Q, &qe) != 0.5f || reinterpret_cast<uintptr_t>(H) % 16) return false;
+  FrontOut o{};
+  o.Gi = Gi;
+  o.hi = hi;
+  hipLaunchKernelGGL((front_end_cpx128_kernel<true>), dim3(n), dim3(64), 0, st, H, y, nullptr, 1.f, 16, 1, Q, o, xg,
+                     list);
   return true;
 }
 

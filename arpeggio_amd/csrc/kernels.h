@@ -71,6 +71,10 @@ hipError_t launch_front_end_sweep(const float* H, const float* y, const float* N
                                   int M, int K, float Q, const FrontOut& o, hipStream_t st);
 void set_front_end_form(int form);
 int front_end_form();  // 2: the frames' H are real lifts of complex channels (front end and decodes read half of H)
+// The MDI sub-problem build at 64 users (D = Mr = 128) on the complex-lift front end with the decisions given; false
+// (nothing launched) for a Q that is not a power of two or a misaligned H.
+bool mdi_build_cpx128(const float* H, const float* y, const int8_t* xg, const int32_t* list, int n, float Q,
+                      int32_t* Gi, int32_t* hi, hipStream_t st);
 // The complex-lift front end (front_end_cpx.hip) for H = [[Re, -Im], [Im, Re]], D = Mr in {32, 64}; false (nothing
 // launched) for other shapes, the expanded Chord inputs, a Q that is not a power of two, or a misaligned H.
 bool front_end_cpx(const float* H, const float* y, const float* N0v, float N0s, int B, int Mr, int D, int M, int K,

@@ -278,8 +278,11 @@ hipError_t launch_mdi_radius(const float* H, const float* y, const int8_t* xg, i
 
 hipError_t launch_mdi_build(const float* H, const float* y, const int8_t* xg, const int32_t* list, int n, int Mr, int D,
                             float Q, int32_t* Gi, int32_t* hi, hipStream_t st) {
-  // up to 32 users: the matrix-core front end in its given-decision form (Gram on MFMA, no inversion)
+  // up to 32 users: the matrix-core front end in its given-decision form (Gram on MFMA, no inversion); 64 users with
+  // the complex-lift form selected: the Hermitian Gram from half of H, likewise
   if (mdi_build_mfma(H, y, xg, list, n, Mr, D, Q, Gi, hi, st)) return hipGetLastError();
+  if (D == 128 && Mr == 128 && front_end_form() == 2 && mdi_build_cpx128(H, y, xg, list, n, Q, Gi, hi, st))
+    return hipGetLastError();
   const size_t lds = sizeof(float) * ((size_t)Mr * D + Mr + D + 2);
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)mdi_build_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

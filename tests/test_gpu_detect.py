@@ -78,14 +78,14 @@ def test_front_end_vs_oracle(orc, fe_form, nt, M, snr, K, form):
         assert (oj['J'].cpu().numpy() == J).all() and (oj['hN'].cpu().numpy() == hN).all()
 
 
-@pytest.mark.parametrize('nt,M,snr', [(32, 16, 23.7), (16, 16, 16.0), (32, 4, 10.0)])
+@pytest.mark.parametrize('nt,M,snr', [(32, 16, 23.7), (16, 16, 16.0), (32, 4, 10.0), (64, 16, 28.0), (64, 4, 12.0)])
 def test_complex_lift_form_against_general(fe_form, nt, M, snr):
     """Form 2 (Hermitian Gram and complex sweep from half of H) against form 0 on the same real-lift frames, learned
     features included: decisions equal but for float near-ties, integers within one unit."""
     from arpeggio_amd.detector import front_end
     from arpeggio_amd.learned import feat_ld, features
     say('complex lift vs general', nt, M, snr)
-    b = frames(nt, M, snr, 4096, seed=77)
+    b = frames(nt, M, snr, 4096 if nt < 64 else 1024, seed=77)
     out = []
     for form in (0, 2):
         fe_form(form)
@@ -105,10 +105,11 @@ def test_complex_lift_form_against_general(fe_form, nt, M, snr):
     np.testing.assert_allclose(c['feat'][same], a['feat'][same], rtol=2e-2, atol=2e-2)
 
 
-@pytest.mark.parametrize('nt,M', [(32, 16), (16, 16), (32, 4)])
+@pytest.mark.parametrize('nt,M', [(32, 16), (16, 16), (32, 4), (64, 16)])
 def test_decode_lift_form_equals_general(fe_form, nt, M):
-    """With form 2 the two-copy decode reads only the first D/2 columns of the real-lift H and rebuilds each row: the
-    distances, decisions and bit errors equal the full-H decode's bit for bit."""
+    """With form 2 the two-copy decodes (one wave at 16 and 32 users, two at 64) read only the first D/2 columns of
+    the real-lift H and rebuild each row: the distances, decisions and bit errors equal the full-H decode's bit for
+    bit."""
     from arpeggio_amd.detector import decode_select, front_end
     say('decode lift', nt, M)
     b = frames(nt, M, 14.0, 2048, seed=5)

@@ -73,12 +73,14 @@ def test_mdi_radius_vs_oracle(orc, mdo):
         np.testing.assert_allclose(d.cpu().numpy(), orc.dist(H, y, xg[:, None, :])[:, 0], rtol=2e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize('form', [0, 2])
 @pytest.mark.parametrize('nt', [16, 32, 64])
-def test_mdi_build_vs_oracle(orc, mdo, nt):
+def test_mdi_build_vs_oracle(orc, mdo, nt, form):
     """Sub-problems around a guess for a frame list: exact integers except within 0.05 of a rounding boundary; frames
-    off the list untouched."""
+    off the list untouched (form 2 at 64 users: the complex-lift build from half of H)."""
     from arpeggio_amd import _lib, frames
-    say('build', nt)
+    say('build', nt, 'form', form)
+    _lib.check(_lib.load().arp_set_front_end_form(form), 'form')
     B, D, M = 40, 2 * nt, 16
     b = frames.draw(933, 18.0, 0, B, nt, M, dev='cuda')
     H, y = b['H'].cpu().numpy(), b['y'].cpu().numpy()
@@ -101,6 +103,7 @@ def test_mdi_build_vs_oracle(orc, mdo, nt):
     assert ((Gg[idx] == Go) | near_g).all() and ((hg[idx] == ho) | near_h).all()
     off = np.setdiff1d(np.arange(B), idx)
     assert (Gg[off] == 7).all() and (hg[off] == 7).all()
+    _lib.check(_lib.load().arp_set_front_end_form(0), 'form')
 
 
 def test_mdi_chain_vs_oracle(orc, mdo):
