@@ -40,6 +40,8 @@ SIGNATURES = {
     'arp_philox_kat': [_P, _P, _i32, _P],
     'arp_set_anneal_waves': [_i32],
     'arp_set_anneals_per_wave': [_i32],
+    'arp_probe_lds': [_P, _i32, _i32, _P],
+    'arp_probe_issue': [_i32, _P, _i32, _i32, _P],
     'arp_set_front_end_form': [_i32],
     'arp_anneal_arpeggio': [_P, _P, _P, _i32, _i32, _i32, _i32, _i32, _i32, _P, _u32, _u32, _i32, _i32, _P, _P, _P,
                             _P],

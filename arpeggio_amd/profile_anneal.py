@@ -8,6 +8,9 @@
         `launches` times in a trace. plain: front end [+ MLP] + anneal + decode. adaptive: the two-stage detector
         (front end [+ MLP], stage-1 anneal and decode, certificate, stage-2 anneal and decode of the open frames).
         --front-end-form: 2 (default, as the bench) the complex-lift front end, 0 the general matrix-core form.
+    python3 -m arpeggio_amd.profile_anneal chord [--frames 4096] [--launches 2]
+        64 users x 64 rx, 16-QAM, Kronecker rho = 0.5, 28 dB: the bench's Chord detection (32 anneals of 256 steps,
+        t 0.2 -> 0.2, q 0.4, kappa 0.3) `launches` times, for the snapshot kernel's counters.
     python3 -m arpeggio_amd.profile_anneal copy
         three device copies of 256 MiB: the known-size reference for scaling FETCH_SIZE / WRITE_SIZE.
     python3 -m arpeggio_amd.profile_anneal traffic FETCH_DB WRITE_DB COPY_FETCH_DB COPY_WRITE_DB [--match arpeggio]
@@ -84,6 +87,21 @@ def traffic(a):
     print(json.dumps(out))
 
 
+def chord(a):
+    import torch
+    from . import _lib, frames
+    from .detector import DetectPipeline
+    _lib.check(_lib.load().arp_set_front_end_form(2), 'front end form')
+    b = frames.draw_device(7, 28.0, 28.0, a.frames, 64, 16, rho_rx=0.5, rho_tx=0.5)
+    pipe = DetectPipeline(a.frames, 128, 128, 16, 32, key=1, policy='chord', cycles=256, T0=0.2, T1=0.2, q=0.4,
+                          kappa=0.3)
+    lt = b['l'].to(torch.int8).contiguous()
+    for _ in range(a.launches):
+        pipe.run(b['H'], b['y'], b['N0'], lt)
+    torch.cuda.synchronize()
+    print('chord launches', a.launches, 'bit errors', int(pipe.biterr.sum()))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest='cmd', required=True)
@@ -98,6 +116,9 @@ def main():
     r.add_argument('--stage1', default='4,24', help='adaptive stage 1: anneals,passes')
     r.add_argument('--c', type=float, default=6.0, help='adaptive certificate constant')
     r.add_argument('--front-end-form', type=int, default=2, choices=(0, 1, 2))
+    c = sub.add_parser('chord')
+    c.add_argument('--frames', type=int, default=4096)
+    c.add_argument('--launches', type=int, default=2)
     sub.add_parser('copy')
     t = sub.add_parser('traffic')
     t.add_argument('dbs', nargs=4)
@@ -106,6 +127,8 @@ def main():
     a = ap.parse_args()
     if a.cmd == 'run':
         run(a)
+    elif a.cmd == 'chord':
+        chord(a)
     elif a.cmd == 'copy':
         copy()
     else:

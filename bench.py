@@ -38,10 +38,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# MI355X: 256 CUs x 4 SIMDs at 2.4 GHz (rocminfo); a wave64 vector instruction holds a SIMD for 4 clocks; LDS
-# delivers 128 B per clock per CU
+# MI355X: 256 CUs x 4 SIMDs at 2.4 GHz (rocminfo); a wave64 vector instruction holds a SIMD for 4 clocks. LDS read
+# bandwidth: measured by arpeggio_amd.probes (profiles/chip_probes.json: conflict-free 16-byte reads from every CU),
+# 128 B per clock per CU where that record is absent
 SIMDS, CLOCK_HZ = 256 * 4, 2.4e9
-LDS_PEAK_GBS = 128 * 256 * CLOCK_HZ / 1e9
+
+
+def _lds_peak():
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'chip_probes.json')
+    if os.path.exists(p):
+        return float(json.load(open(p))['lds']['GBs'])
+    return 128 * 256 * CLOCK_HZ / 1e9
+
+
+LDS_PEAK_GBS = _lds_peak()
 MFMA_FP32_TFLOPS = 157.3
 NT, M = 32, 16
 METRIC = 'detections/s at fixed BER (32u 16-QAM), learned-start vs plain annealer'

@@ -53,6 +53,12 @@ int arp_set_anneals_per_wave(int apw);
  * results agree to float32 rounding. Process-wide. */
 int arp_set_front_end_form(int form);
 
+/* Chip probes (no reference counterpart; measurement only). arp_probe_lds: conflict-free 16-byte LDS reads, `iters`
+ * rounds of 8 KiB per wave64, 256-thread workgroups; arp_probe_issue: mode 0 eight independent FP32 MFMAs (16x16x4)
+ * per iteration, mode 1 sixty-four independent FMAs, mode 2 both in each wave. out: workgroups * 256 floats. */
+int arp_probe_lds(float *out, int workgroups, int iters, void *stream);
+int arp_probe_issue(int mode, float *out, int workgroups, int iters, void *stream);
+
 /* Philox4x32-10 known-answer kernel: in (n, 6) uint32 rows (counter[4], key[2]) -> out (n, 4).
  * Replaces the reference's device KAT `philox_kat` (arpeggio/mimo/chord_kernel.cu:26-32), driven by
  * Kernel.philox_kat (arpeggio/mimo/chordfx.py:180-187). */
