@@ -85,6 +85,8 @@ def parse():
     ap.add_argument('--no-mdi', action='store_true', help='skip the MDI wrapper against the reference fixtures')
     ap.add_argument('--mdi-batches', type=int, default=60, help='32-user fixture batches (1,000 frames each)')
     ap.add_argument('--mdi-large', type=int, default=16384, help='frames of the MDI throughput batch')
+    ap.add_argument('--stage2-copies', type=int, default=None,
+                    help='adaptive stage 2 anneals (default: picked on calibration frames by the equivalence rule)')
     ap.add_argument('--front-end-form', type=int, default=2, choices=(0, 1, 2),
                     help='2: the complex-lift front end (the frames are real lifts of complex channels), 0: general')
     return ap.parse_args()
@@ -145,13 +147,14 @@ def calibrate(args):
 
 
 C_GRID = (0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0)   # the reference's grid of stop constants (a56an.py:10)
+S2_GRID = (16, 12, 8)                               # stage-2 anneal counts tried, the plain count first
 EQUIV_LIMIT = 1.10                             # the reference's BER-equivalence cut-off (README.md:171)
 
 
-def make_adaptive(B, args, start, model=None, c=3.0):
+def make_adaptive(B, args, start, model=None, c=3.0, r2=None):
     from arpeggio_amd.detector import AdaptivePipeline
     R1, L1 = (int(x) for x in args.stage1.split(','))
-    return AdaptivePipeline(B, 2 * NT, 2 * NT, M, R1=R1, L1=L1, R2=args.copies, L2=args.passes, c=c, start=start,
+    return AdaptivePipeline(B, 2 * NT, 2 * NT, M, R1=R1, L1=L1, R2=r2 or args.copies, L2=args.passes, c=c, start=start,
                             T0=0.1, T1=0.1, q=0.0, key=key_of(args.seed), inst0=0,
                             model=model if start == 'learned' else None)
 
@@ -179,6 +182,32 @@ def pick_c(args, snr, model):
         if hi is not None and hi <= EQUIV_LIMIT:
             best = c
     return best, rows
+
+
+def pick_stage2(args, snr, model, c):
+    """Stage 2's anneal count by the reference's equivalence rule (README.md:171; a56an.py:74-95) on the calibration
+    frames of pick_c: the smallest count of S2_GRID whose learned adaptive detector (stop constant c, stage 1 as
+    configured, stage 2 at the plain pass budget) has a paired BER ratio to the plain detector with an upper 95 % bound
+    <= 1.10; the grid is walked from the plain count down and stops at the first miss."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.stats import boot_ratio
+    Bc = args.calib_frames
+    b = frames.draw(args.seed, snr, 1, Bc, NT, M, dev='cuda')
+    base = frame_errors(make_pipe(Bc, args, args.passes, 'rand'), b)
+    rows, best = [], None
+    for r2 in S2_GRID:
+        pipe = make_adaptive(Bc, args, 'learned', model, c, r2)
+        be = frame_errors(pipe, b)
+        r, lo, hi = boot_ratio(be, base)
+        rows.append(dict(copies=r2, ratio=r, ci=[lo, hi]))
+        log('stage 2 with %d anneals: BER ratio to plain %.3f [%.3f, %.3f]' % (r2, r, lo, hi))
+        del pipe
+        if hi is None or hi > EQUIV_LIMIT:
+            break
+        best = r2
+    # no count passes on these frames (the interval of the plain count itself can straddle the cut-off): the plain
+    # count, whose equivalence the measurement frames then test
+    return best or args.copies, rows
 
 
 def stage_ms(pipe, b, reps=5):
@@ -588,9 +617,14 @@ def main():
         else:
             c, crows = args.c, None
         log('stop constant c = %.1f' % c)
+        if args.stage2_copies is None:
+            r2, srows = pick_stage2(args, snr, model, c)
+        else:
+            r2, srows = args.stage2_copies, None
+        log('stage 2: %d anneals x %d passes' % (r2, L))
         recs = {}
         for start in ('learned', 'rand'):
-            ap = make_adaptive(B, args, start, model, c)
+            ap = make_adaptive(B, args, start, model, c, r2)
             be = frame_errors(ap, b)
             r, lo, hi = boot_ratio(be, be_plain)
             open_frac = int(ap.count.item()) / B
@@ -599,7 +633,7 @@ def main():
             t = time_pipe(ap, b, args.steps, args.warmup, args.warm_seconds)
             recs[start] = dict(value=t['value'], ms_per_step=t['ms_per_step'], gpu_ms_per_step=t['gpu_ms_per_step'],
                                ber=t['ber'], ratio_to_plain=r, ci=[lo, hi], ber_matched=bool(hi <= EQUIV_LIMIT),
-                               open_fraction=open_frac, stage1=args.stage1, stage2='%d,%d' % (R, L), c=c,
+                               open_fraction=open_frac, stage1=args.stage1, stage2='%d,%d' % (r2, L), c=c,
                                speedup_vs_plain=t['value'] / tp['value'], stages=stage_ms(ap, b))
             if start == 'learned':
                 lp = ap
@@ -620,6 +654,9 @@ def main():
         out['learned_no_stop'] = recs['learned_no_stop']
         out['stop_constant'] = dict(c=c, grid=crows, rule='largest c with paired BER ratio to no stop <= 1.10 at the '
                                     'upper 95 % bound (arpeggio/sim/a56an.py:109-118)')
+        out['stage2_budget'] = dict(copies=r2, passes=L, grid=srows,
+                                    rule='smallest stage-2 anneal count with paired BER ratio to plain <= 1.10 at the '
+                                    'upper 95 % bound on the calibration frames (README.md:171)')
     if matched:
         out.update(value=tl['value'], ms_per_step=tl['ms_per_step'])
         out['speedup_vs_plain'] = tl['value'] / tp['value']
@@ -642,8 +679,9 @@ def main():
                                           'arpeggio_fx_kernel<2,1,0,true,64>')
     out['plain'] = plain_rec
     out['config'] = dict(workload='one-stage dDI detection, 32x32 16-QAM i.i.d. Rayleigh, %d frames/step; plain: R=%d '
-                                  'anneals x %d passes; learned: adaptive, stage 1 %s (anneals,passes), stage 2 the '
-                                  'plain budget on open frames; %s step timed' % (B, R, L, args.stage1, timed),
+                                  'anneals x %d passes; learned: adaptive, stage 1 %s (anneals,passes), stage 2 %d '
+                                  'anneals (picked by the equivalence rule) x %d passes on open frames; %s step timed' % (
+                                      B, R, L, args.stage1, out.get('stage2_budget', {}).get('copies', R), L, timed),
                          frames=B, users=NT, rx=NT, M=M, copies=R, plain_passes=L, stage1=args.stage1,
                          policy='arpeggio', snr_db=snr, target_ber=args.target_ber, calibration=calib,
                          T0=0.1, T1=0.1, q=0.0,
