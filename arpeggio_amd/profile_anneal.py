@@ -3,7 +3,7 @@
 
     python3 -m arpeggio_amd.profile_anneal run [--start rand|learned] [--passes 64] [--frames 65536] [--copies 16]
                                                [--launches 3] [--snr 23.68] [--pipeline plain|adaptive]
-                                               [--stage1 4,24] [--c 6] [--front-end-form 2]
+                                               [--stage1 4,24] [--c 6] [--front-end-form 2] [--gate]
         draws the 32x32 16-QAM frames once, then runs the full step `launches` times; every kernel of the path appears
         `launches` times in a trace. plain: front end [+ MLP] + anneal + decode. adaptive: the two-stage detector
         (front end [+ MLP], stage-1 anneal and decode, certificate, stage-2 anneal and decode of the open frames).
@@ -41,9 +41,10 @@ def run(a):
         from .detector import AdaptivePipeline
         R1, L1 = (int(x) for x in a.stage1.split(','))
         pipe = AdaptivePipeline(a.frames, 64, 64, 16, R1=R1, L1=L1, R2=a.copies, L2=a.passes, c=a.c, start=a.start,
-                                key=key, model=model)
+                                key=key, model=model, gate=a.gate)
     else:
-        pipe = DetectPipeline(a.frames, 64, 64, 16, a.copies, cycles=a.passes, start=a.start, key=key, model=model)
+        pipe = DetectPipeline(a.frames, 64, 64, 16, a.copies, cycles=a.passes, start=a.start, key=key, model=model,
+                              gate=a.gate)
     for i in range(a.launches):
         pipe.run(b['H'], b['y'], b['N0'], lt)
         torch.cuda.synchronize()
@@ -116,6 +117,7 @@ def main():
     r.add_argument('--stage1', default='4,24', help='adaptive stage 1: anneals,passes')
     r.add_argument('--c', type=float, default=6.0, help='adaptive certificate constant')
     r.add_argument('--front-end-form', type=int, default=2, choices=(0, 1, 2))
+    r.add_argument('--gate', action='store_true', help='learned start with the energy gate (the bench: without)')
     c = sub.add_parser('chord')
     c.add_argument('--frames', type=int, default=4096)
     c.add_argument('--launches', type=int, default=2)

@@ -85,6 +85,9 @@ def parse():
     ap.add_argument('--no-mdi', action='store_true', help='skip the MDI wrapper against the reference fixtures')
     ap.add_argument('--mdi-batches', type=int, default=60, help='32-user fixture batches (1,000 frames each)')
     ap.add_argument('--mdi-large', type=int, default=16384, help='frames of the MDI throughput batch')
+    ap.add_argument('--gate', action='store_true',
+                    help='headline learned start with the energy gate (fallback to the random start per anneal); '
+                         'the gated detector is measured beside it either way')
     ap.add_argument('--stage2-copies', type=int, default=None,
                     help='adaptive stage 2 anneals (default: picked on calibration frames by the equivalence rule)')
     ap.add_argument('--front-end-form', type=int, default=2, choices=(0, 1, 2),
@@ -114,7 +117,7 @@ def key_of(seed):
 def make_pipe(B, args, L, start, model=None):
     from arpeggio_amd.detector import DetectPipeline
     return DetectPipeline(B, 2 * NT, 2 * NT, M, args.copies, policy='arpeggio', cycles=L, T0=0.1, T1=0.1, q=0.0,
-                          start=start, key=key_of(args.seed), inst0=0, model=model)
+                          start=start, key=key_of(args.seed), inst0=0, model=model, gate=args.gate)
 
 
 def frame_errors(pipe, b):
@@ -151,12 +154,13 @@ S2_GRID = (16, 12, 8)                               # stage-2 anneal counts trie
 EQUIV_LIMIT = 1.10                             # the reference's BER-equivalence cut-off (README.md:171)
 
 
-def make_adaptive(B, args, start, model=None, c=3.0, r2=None):
+def make_adaptive(B, args, start, model=None, c=3.0, r2=None, gate=None):
     from arpeggio_amd.detector import AdaptivePipeline
     R1, L1 = (int(x) for x in args.stage1.split(','))
     return AdaptivePipeline(B, 2 * NT, 2 * NT, M, R1=R1, L1=L1, R2=r2 or args.copies, L2=args.passes, c=c, start=start,
                             T0=0.1, T1=0.1, q=0.0, key=key_of(args.seed), inst0=0,
-                            model=model if start == 'learned' else None)
+                            model=model if start == 'learned' else None,
+                            gate=args.gate if gate is None else gate)
 
 
 def pick_c(args, snr, model):
@@ -623,19 +627,21 @@ def main():
             r2, srows = args.stage2_copies, None
         log('stage 2: %d anneals x %d passes' % (r2, L))
         recs = {}
-        for start in ('learned', 'rand'):
-            ap = make_adaptive(B, args, start, model, c, r2)
+        for name, start, gate in (('learned', 'learned', args.gate), ('learned_other_gate', 'learned', not args.gate),
+                                  ('rand', 'rand', False)):
+            ap = make_adaptive(B, args, start, model, c, r2, gate)
             be = frame_errors(ap, b)
             r, lo, hi = boot_ratio(be, be_plain)
             open_frac = int(ap.count.item()) / B
-            log('adaptive, %s start: BER %.3e, ratio to plain %.3f [%.3f, %.3f], open %.3f' % (
-                start, be.sum() / nbits, r, lo, hi, open_frac))
+            log('adaptive, %s: BER %.3e, ratio to plain %.3f [%.3f, %.3f], open %.3f' % (
+                name, be.sum() / nbits, r, lo, hi, open_frac))
             t = time_pipe(ap, b, args.steps, args.warmup, args.warm_seconds)
-            recs[start] = dict(value=t['value'], ms_per_step=t['ms_per_step'], gpu_ms_per_step=t['gpu_ms_per_step'],
+            recs[name] = dict(value=t['value'], ms_per_step=t['ms_per_step'], gpu_ms_per_step=t['gpu_ms_per_step'],
                                ber=t['ber'], ratio_to_plain=r, ci=[lo, hi], ber_matched=bool(hi <= EQUIV_LIMIT),
                                open_fraction=open_frac, stage1=args.stage1, stage2='%d,%d' % (r2, L), c=c,
-                               speedup_vs_plain=t['value'] / tp['value'], stages=stage_ms(ap, b))
-            if start == 'learned':
+                               speedup_vs_plain=t['value'] / tp['value'], stages=stage_ms(ap, b),
+                               gate=bool(gate) if start == 'learned' else None)
+            if name == 'learned':
                 lp = ap
                 tl = t
             else:
@@ -651,6 +657,7 @@ def main():
         matched = learned['ber_matched']
         out['learned'] = learned
         out['adaptive_random_start'] = recs['rand']
+        out['learned_gated' if not args.gate else 'learned_ungated'] = recs['learned_other_gate']
         out['learned_no_stop'] = recs['learned_no_stop']
         out['stop_constant'] = dict(c=c, grid=crows, rule='largest c with paired BER ratio to no stop <= 1.10 at the '
                                     'upper 95 % bound (arpeggio/sim/a56an.py:109-118)')
