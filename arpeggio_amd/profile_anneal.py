@@ -3,7 +3,7 @@
 
     python3 -m arpeggio_amd.profile_anneal run [--start rand|learned] [--passes 64] [--frames 65536] [--copies 16]
                                                [--launches 3] [--snr 23.68] [--pipeline plain|adaptive]
-                                               [--stage1 4,24] [--c 6] [--front-end-form 2] [--gate]
+                                               [--stage1 4,40] [--c 6] [--front-end-form 2] [--gate]
         draws the 32x32 16-QAM frames once, then runs the full step `launches` times; every kernel of the path appears
         `launches` times in a trace. plain: front end [+ MLP] + anneal + decode. adaptive: the two-stage detector
         (front end [+ MLP], stage-1 anneal and decode, certificate, stage-2 anneal and decode of the open frames).
@@ -114,7 +114,7 @@ def main():
     r.add_argument('--launches', type=int, default=3)
     r.add_argument('--snr', type=float, default=23.68)
     r.add_argument('--pipeline', default='plain', choices=('plain', 'adaptive'))
-    r.add_argument('--stage1', default='4,24', help='adaptive stage 1: anneals,passes')
+    r.add_argument('--stage1', default='4,40', help='adaptive stage 1: anneals,passes (the bench\'s pick)')
     r.add_argument('--c', type=float, default=6.0, help='adaptive certificate constant')
     r.add_argument('--front-end-form', type=int, default=2, choices=(0, 1, 2))
     r.add_argument('--gate', action='store_true', help='learned start with the energy gate (the bench: without)')

@@ -68,7 +68,8 @@ def parse():
     ap.add_argument('--target-ber', type=float, default=1e-3)
     ap.add_argument('--copies', type=int, default=16)
     ap.add_argument('--passes', type=int, default=64, help='plain annealer pass budget')
-    ap.add_argument('--stage1', default='4,24', help='adaptive stage 1: anneals,passes')
+    ap.add_argument('--stage1', default=None, help='adaptive stage 1: anneals,passes (default: picked on calibration '
+                    'frames, the fastest equivalent budget of S1_GRID)')
     ap.add_argument('--c', type=float, default=None, help='certificate constant (default: picked on calibration '
                     'frames by the reference rule)')
     ap.add_argument('--weights', default=None)
@@ -151,6 +152,9 @@ def calibrate(args):
 
 C_GRID = (0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0)   # the reference's grid of stop constants (a56an.py:10)
 S2_GRID = (16, 12, 8)                               # stage-2 anneal counts tried, the plain count first
+# stage-1 budgets (anneals, passes) tried; every pass count is a whole number of Philox blocks (8 passes at P = 2)
+S1_GRID = ((2, 16), (2, 32), (3, 24), (3, 32), (4, 16), (4, 24), (4, 32), (4, 40), (6, 24))
+S1_DEFAULT = '4,24'                                 # stage 1 while the stop constant is picked
 EQUIV_LIMIT = 1.10                             # the reference's BER-equivalence cut-off (README.md:171)
 
 
@@ -186,6 +190,35 @@ def pick_c(args, snr, model):
         if hi is not None and hi <= EQUIV_LIMIT:
             best = c
     return best, rows
+
+
+def pick_stage1(args, snr, model, c):
+    """Stage 1's budget (anneals x passes, S1_GRID) on the calibration frames of pick_c: the fastest learned adaptive
+    detector (stop constant c, stage 2 at the plain budget) whose paired BER ratio to the plain detector has an upper
+    95 % bound <= 1.10 (README.md:171). Each candidate's step is timed on those frames after a warm-up."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.stats import boot_ratio
+    Bc = args.calib_frames
+    b = frames.draw(args.seed, snr, 1, Bc, NT, M, dev='cuda')
+    base = frame_errors(make_pipe(Bc, args, args.passes, 'rand'), b)
+    keep = args.stage1
+    rows, best = [], None
+    for r1, l1 in S1_GRID:
+        args.stage1 = '%d,%d' % (r1, l1)
+        pipe = make_adaptive(Bc, args, 'learned', model, c)
+        be = frame_errors(pipe, b)
+        r, lo, hi = boot_ratio(be, base)
+        ms = time_pipe(pipe, b, 10, 2, 0.3)['ms_per_step']
+        ok = hi is not None and hi <= EQUIV_LIMIT
+        rows.append(dict(anneals=r1, passes=l1, ratio=r, ci=[lo, hi], open=int(pipe.count.item()) / Bc, ms=ms,
+                         equivalent=bool(ok)))
+        log('stage 1 %d x %d: BER ratio to plain %.3f [%.3f, %.3f], %.3f ms per %d frames' % (r1, l1, r, lo, hi, ms,
+                                                                                              Bc))
+        del pipe
+        if ok and (best is None or ms < best[1]):
+            best = (args.stage1, ms)
+    args.stage1 = best[0] if best else keep
+    return args.stage1, rows
 
 
 def pick_stage2(args, snr, model, c):
@@ -611,6 +644,9 @@ def main():
                      passes=L, copies=R, start='rand', ber=float(be_plain.sum()) / nbits, anneal_ms=tp['anneal_ms'],
                      stages=stage_ms(plain, b))
     matched = False
+    s1_given = args.stage1 is not None
+    if not s1_given:
+        args.stage1 = S1_DEFAULT   # for pick_c; pick_stage1 then replaces it
     if not args.plain_only:
         from arpeggio_amd.learned import DEFAULT_WEIGHTS, WarmStartMLP
         model = WarmStartMLP.load(args.weights or DEFAULT_WEIGHTS)
@@ -621,6 +657,10 @@ def main():
         else:
             c, crows = args.c, None
         log('stop constant c = %.1f' % c)
+        s1rows = None
+        if not s1_given:
+            _, s1rows = pick_stage1(args, snr, model, c)
+        log('stage 1: %s (anneals,passes)' % args.stage1)
         if args.stage2_copies is None:
             r2, srows = pick_stage2(args, snr, model, c)
         else:
@@ -661,6 +701,9 @@ def main():
         out['learned_no_stop'] = recs['learned_no_stop']
         out['stop_constant'] = dict(c=c, grid=crows, rule='largest c with paired BER ratio to no stop <= 1.10 at the '
                                     'upper 95 % bound (arpeggio/sim/a56an.py:109-118)')
+        out['stage1_budget'] = dict(stage1=args.stage1, grid=s1rows,
+                                    rule='fastest stage-1 budget (anneals x passes) with paired BER ratio to plain <= '
+                                    '1.10 at the upper 95 % bound on the calibration frames (README.md:171)')
         out['stage2_budget'] = dict(copies=r2, passes=L, grid=srows,
                                     rule='smallest stage-2 anneal count with paired BER ratio to plain <= 1.10 at the '
                                     'upper 95 % bound on the calibration frames (README.md:171)')
