@@ -6,8 +6,8 @@ A step is one pass of the kernel path over the batch with H, y and the true leve
   plain:    front end (LMMSE + dDI Ising build + fixed point) -> fixed-point Arpeggio (first accept), 16 anneals x 64
             passes from the random start -> decode/select -> bit errors (the reference's 32-user pick
             abl_far_32_0.1_0.1_0_rand_*: 32 sweeps x 2 copies, T = 0.1, q = 0)
-  learned:  the adaptive detector: front end with the warm-start features -> MLP -> start thresholds -> stage 1, 4
-            learned-start anneals x 24 passes for every frame -> decode -> certificate ||y - Hx||^2 <= N0 (Nr + c
+  learned:  the adaptive detector: front end with the warm-start features -> MLP -> start thresholds -> stage 1,
+            learned-start anneals (budget picked in step 3) for every frame -> decode -> certificate ||y - Hx||^2 <= N0 (Nr + c
             sqrt(Nr)) (the reference's first-pass stop rule, arpeggio/sim/a56an.py:58-67) -> stage 2, the plain
             budget (16 anneals x 64 passes, new streams) for the open frames only -> decode against stage 1's choice
 
@@ -16,9 +16,11 @@ Procedure (fixed-BER comparison):
      SNRs; --snr skips this);
   2. c is picked by the reference's rule (a56an.py:109-118) on the calibration frames at that SNR: the largest c of
      its grid whose paired-bootstrap BER ratio to the same detector without the stop has an upper 95 % bound <= 1.10;
-  3. on the measurement frames, the learned detector's BER must match plain's: paired-bootstrap ratio (a56an.py:74-95)
+  3. stage 1's budget (anneals x passes) on the same frames: the fastest of S1_GRID whose paired BER ratio to plain
+     has an upper 95 % bound <= 1.10; then stage 2's anneal count by the same rule (S2_GRID);
+  4. on the measurement frames, the learned detector's BER must match plain's: paired-bootstrap ratio (a56an.py:74-95)
      with an upper 95 % bound <= 1.10 (the reference's equivalence cut-off, README.md:171);
-  4. both pipelines are timed over K steps; value = learned detections/s, plain.value and speedup beside it, and the
+  5. both pipelines are timed over K steps; value = learned detections/s, plain.value and speedup beside it, and the
      same adaptive detector from the random start (what the network adds) and the learned start without the stop.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--snr DB] [--frames B] [--no-cpu] [--with-chord]
@@ -552,6 +554,58 @@ def batched(args):
 SNR_64 = 28.0                                       # where the annealers beat LMMSE on this channel
 SCHED_64 = dict(T0=0.08, T1=0.03, q=0.0)            # first accept, tuned at 26-28 dB on 4,096 frames
 CHORD_64 = dict(T0=0.2, T1=0.2, q=0.4, kappa=0.3)   # Chord re-tuned on the same frames (256 steps)
+S1_GRID_64 = ((4, 16), (8, 16), (8, 32), (8, 64), (16, 32))   # 64-user stage-1 budgets (anneals, passes)
+CALIB_64 = 16384                                    # 64-user calibration frames (a separate draw)
+
+
+def adaptive64(args, model, c, stage1, L2, B, nt=64):
+    from arpeggio_amd.detector import AdaptivePipeline
+    R1, L1 = stage1
+    return AdaptivePipeline(B, 2 * nt, 2 * nt, 16, R1=R1, L1=L1, R2=32, L2=L2, c=c, start='learned',
+                            key=key_of(args.seed), inst0=0, model=model, gate=False, **SCHED_64)
+
+
+def pick_adaptive64(args, model, L2, nt=64):
+    """The 64-user adaptive detector's stop constant and stage-1 budget on calibration frames (draw seed + 65, 28 dB):
+    c by the reference's rule against the same detector without the stop (a56an.py:109-118), at stage 1 8 x 32; then
+    the fastest stage-1 budget of S1_GRID_64 whose paired BER ratio to plain at 32 x 128 has an upper 95 % bound
+    <= 1.10 (README.md:171)."""
+    import torch
+    from arpeggio_amd import frames
+    from arpeggio_amd.detector import DetectPipeline
+    from arpeggio_amd.stats import boot_ratio
+    Bc = CALIB_64
+    bc = frames.draw_device(args.seed + 65, SNR_64, SNR_64, Bc, nt, 16, rho_rx=0.5, rho_tx=0.5)
+    pl = DetectPipeline(Bc, 2 * nt, 2 * nt, 16, 32, key=key_of(args.seed), inst0=0, policy='arpeggio', cycles=128,
+                        start='rand', **SCHED_64)
+    ref = frame_errors(pl, bc)
+    del pl
+    pipe = adaptive64(args, model, 0.0, (8, 32), L2, Bc, nt)
+    pipe.thr_scale = -1.0                       # no stop: every frame is open
+    base = frame_errors(pipe, bc)
+    c, crows = C_GRID[0], []
+    for cc in C_GRID:
+        pipe.c, pipe.thr_scale = cc, nt + cc * nt ** 0.5
+        r, lo, hi = boot_ratio(frame_errors(pipe, bc), base)
+        crows.append(dict(c=cc, ratio=r, ci=[lo, hi]))
+        if hi is not None and hi <= EQUIV_LIMIT:
+            c = cc
+    del pipe
+    best, srows = None, []
+    for s1 in S1_GRID_64:
+        pipe = adaptive64(args, model, c, s1, L2, Bc, nt)
+        r, lo, hi = boot_ratio(frame_errors(pipe, bc), ref)
+        ms = time_pipe(pipe, bc, 5, 1, 0.3)['ms_per_step']
+        ok = hi is not None and hi <= EQUIV_LIMIT
+        srows.append(dict(anneals=s1[0], passes=s1[1], ratio=r, ci=[lo, hi], open=int(pipe.count.item()) / Bc, ms=ms,
+                          equivalent=bool(ok)))
+        log('64 users, stage 1 %d x %d: BER ratio to plain 128 %.3f [%.3f, %.3f], %.2f ms per %d frames' % (
+            s1[0], s1[1], r, lo, hi, ms, Bc))
+        del pipe
+        torch.cuda.empty_cache()
+        if ok and (best is None or ms < best[1]):
+            best = (s1, ms)
+    return c, (best[0] if best else (8, 32)), dict(c_grid=crows, stage1_grid=srows, frames=Bc)
 
 
 def users64(args):
@@ -559,7 +613,9 @@ def users64(args):
     the annealers beat the LMMSE decision), 2^16 frames, 32 anneals per frame, first accept with T 0.08 -> 0.03: plain
     (random start) at 64 and 128 passes; the learned start (the 64-user network) at 32 and 64 passes, each with its
     paired BER ratio to plain at 128 passes, and the speedup of the smallest learned budget whose upper 95 % bound is
-    <= 1.10; Chord re-tuned (256 steps, t 0.2, q 0.4, kappa 0.3) on 4,096 of the frames; the LMMSE decision's BER."""
+    <= 1.10; the adaptive two-stage detector (stage 2 = that learned budget; c and stage 1 picked on calibration
+    frames by pick_adaptive64) with its ratio and speedup; Chord re-tuned (256 steps, t 0.2, q 0.4, kappa 0.3) on 4,096
+    of the frames; the LMMSE decision's BER."""
     import torch
     from arpeggio_amd import frames
     from arpeggio_amd.detector import DetectPipeline, front_end
@@ -595,6 +651,21 @@ def users64(args):
     log('64 users at %.0f dB: LMMSE %.3e, plain 64/128 passes %.3e / %.3e, learned 32/64 %.3e / %.3e' % (
         SNR_64, lmmse, out['plain_64']['ber'], out['plain_128']['ber'], out['learned_32']['ber'],
         out['learned_64']['ber']))
+    # the two-stage detector: stage 2 is the one-stage learned budget that matched plain 128 (64 passes otherwise)
+    L2 = out[chosen]['passes'] if chosen else 64
+    c64, s1, picks = pick_adaptive64(args, model, L2, nt)
+    pipe = adaptive64(args, model, c64, s1, L2, B, nt)
+    _, be = pipe.run(b['H'], b['y'], b['N0'], lt)
+    ea = be.cpu().numpy().astype(np.int64)
+    r, lo, hi = boot_ratio(ea, ref)
+    t = time_pipe(pipe, b, args.steps, args.warmup, args.warm_seconds)
+    out['adaptive'] = dict(value=t['value'], ms_per_step=t['ms_per_step'], ber=float(ea.sum()) / nbits,
+                           ratio_to_plain_128=r, ci=[lo, hi], ber_matched=bool(hi <= EQUIV_LIMIT),
+                           open_fraction=int(pipe.count.item()) / B, stage1='%d,%d' % s1, stage2='32,%d' % L2, c=c64,
+                           start='learned', stages=stage_ms(pipe, b, reps=3), picks=picks)
+    log('64 users, adaptive %d x %d then 32 x %d (c = %.1f): BER %.3e, ratio to plain 128 %.3f [%.3f, %.3f], %.2f ms'
+        % (s1[0], s1[1], L2, c64, out['adaptive']['ber'], r, lo, hi, t['ms_per_step']))
+    del pipe
     Bc = min(B, 4096)
     bc = {k: (v[:Bc] if torch.is_tensor(v) and v.dim() > 0 and v.shape[0] == B else v) for k, v in b.items()}
     pipe = DetectPipeline(Bc, 2 * nt, 2 * nt, 16, 32, key=key_of(args.seed), inst0=0, policy='chord', cycles=256,
@@ -608,6 +679,9 @@ def users64(args):
     if chosen:
         res['learned_fixed_ber'] = dict(budget=chosen, value=out[chosen]['value'],
                                         speedup_vs_plain_128=out[chosen]['value'] / out['plain_128']['value'])
+    if out['adaptive']['ber_matched']:
+        res['adaptive_fixed_ber'] = dict(value=out['adaptive']['value'],
+                                         speedup_vs_plain_128=out['adaptive']['value'] / out['plain_128']['value'])
     return res
 
 

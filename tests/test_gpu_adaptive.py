@@ -26,13 +26,30 @@ def model():
                                                       ('learned', 3, 10, 5, 14, 4.0, 18.0)])
 def test_adaptive_vs_oracle(orc, model, start, R1, L1, R2, L2, c, snr):
     from arpeggio_amd import frames
-    from arpeggio_amd.detector import AdaptivePipeline
     say('adaptive', start, R1, L1, R2, L2, c, snr)
-    B, nt, M, key = 192, 32, 16, 0x5151AB
-    b = frames.draw(808, snr, 0, B, nt, M, dev='cuda')
+    B, nt, M = 192, 32, 16
+    check_adaptive(orc, model, start, R1, L1, R2, L2, c, frames.draw(808, snr, 0, B, nt, M, dev='cuda'), nt, M, {})
+
+
+@pytest.mark.parametrize('start,R1,L1,R2,L2,c', [('learned', 4, 16, 8, 24, 6.0), ('rand', 2, 8, 4, 16, 3.0)])
+def test_adaptive_vs_oracle_64_users(orc, start, R1, L1, R2, L2, c):
+    """64 users x 64 rx (D = 128: the Gram past 64 KiB of LDS, two coordinates per lane), Kronecker-correlated
+    channel at 28 dB, the 64-user network and schedule T 0.08 -> 0.03."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.learned import WEIGHTS_64, WarmStartMLP
+    say('adaptive 64 users', start, R1, L1, R2, L2, c)
+    B, nt, M = 40, 64, 16
+    b = frames.draw_device(977, 28.0, 28.0, B, nt, M, rho_rx=0.5, rho_tx=0.5)
+    m64 = WarmStartMLP.load(WEIGHTS_64)
+    check_adaptive(orc, m64, start, R1, L1, R2, L2, c, b, nt, M, dict(T0=0.08, T1=0.03, q=0.0))
+
+
+def check_adaptive(orc, model, start, R1, L1, R2, L2, c, b, nt, M, sched):
+    from arpeggio_amd.detector import AdaptivePipeline
+    B, key = b['H'].shape[0], 0x5151AB
     lt = b['l'].to(torch.int8).contiguous()
     pipe = AdaptivePipeline(B, 2 * nt, 2 * nt, M, R1=R1, L1=L1, R2=R2, L2=L2, c=c, start=start, key=key, inst0=0,
-                            model=model if start == 'learned' else None)
+                            model=model if start == 'learned' else None, **sched)
     lhat, be = pipe.run(b['H'], b['y'], b['N0'], lt)
     torch.cuda.synchronize()
     G, h = pipe.Gi.cpu().numpy(), pipe.hi.cpu().numpy()
@@ -55,7 +72,8 @@ def test_adaptive_vs_oracle(orc, model, start, R1, L1, R2, L2, c, snr):
     np.testing.assert_allclose(d1, dd1.min(1), rtol=1e-4, atol=1e-3)
     # certificate: the open set is exactly the frames with d1 > N0 (Nr + c sqrt(Nr)) (on the GPU's own distances)
     open_ = pipe.open_frames().numpy()
-    want = np.flatnonzero(~(d1 <= np.float32(b['N0']) * np.float32(pipe.thr_scale)))
+    n0 = b['N0'].cpu().numpy().astype(np.float32) if torch.is_tensor(b['N0']) else np.float32(b['N0'])
+    want = np.flatnonzero(~(d1 <= n0 * np.float32(pipe.thr_scale)))
     assert np.array_equal(np.sort(open_), want)
     say('open frames', len(want), 'of', B)
     n = int(pipe.count.item())
