@@ -458,6 +458,8 @@ __global__ __launch_bounds__(128) void decode_w128_kernel(const float* __restric
     Lc[r * DP + t] = l;
   }
   __syncthreads();
+  // all DP terms: past D both hr (loaded as zero) and the levels (written as zero) are zero, and fma(0, 0, s) = s, so
+  // the sum is the D-term sum bit for bit, without a per-term bound test
   for (int r = 0; r < C; ++r) {
     const float* lc = Lc + r * DP;
     float sacc = 0.f;
@@ -465,8 +467,7 @@ __global__ __launch_bounds__(128) void decode_w128_kernel(const float* __restric
     for (int jj = 0; jj < DP; jj += 4) {
       const floatx4 lv = *reinterpret_cast<const floatx4*>(lc + jj);
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (jj + e < D) sacc = __builtin_fmaf(hr[jj + e], lv[e], sacc);
+      for (int e = 0; e < 4; ++e) sacc = __builtin_fmaf(hr[jj + e], lv[e], sacc);
     }
     const float e = yi - sacc;
     float part = row ? e * e : 0.f;
