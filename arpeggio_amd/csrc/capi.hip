@@ -304,8 +304,8 @@ extern "C" int arp_anneal_arpeggio_batched(const arp_batch_class* classes, int n
 extern "C" int arp_mlp_start_fused(const uint16_t* X, int M, int F, const uint16_t* W1t, const float* b1,
                                    const uint16_t* W2t, const float* b2, const uint16_t* W3t, const float* b3, int H,
                                    int N, float inv_temp, uint32_t* thr, void* stream) {
-  if (M < 0 || F < 32 || F % 32 || F > 504)
-    return fail(ARP_E_ARG, "mlp_start_fused: F = %d, need a multiple of 32 in 32..504 (M = %d)", F, M);
+  if (M < 0 || F < 32 || F % 32 || F > 1016)
+    return fail(ARP_E_ARG, "mlp_start_fused: F = %d, need a multiple of 32 in 32..1016 (M = %d)", F, M);
   if (H != 256) return fail(ARP_E_ARG, "mlp_start_fused: hidden width %d, the fused kernel is built for 256", H);
   if (N < 64 || N > 256 || N % 64) return fail(ARP_E_ARG, "mlp_start_fused: N = %d, need 64, 128, 192 or 256", N);
   if (M == 0) return ok();

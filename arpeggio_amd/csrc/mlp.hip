@@ -285,8 +285,14 @@ hipError_t launch_mlp_start_fused(const uint16_t* X, int M, int F, const uint16_
                                   const uint16_t* W2t, const float* b2, const uint16_t* W3t, const float* b3, int N,
                                   float inv_temp, uint32_t* thr, hipStream_t st) {
   const int lda = (F > 256 ? F : 256) + 8;
-  const size_t lds = sizeof(uint16_t) * 64 * (size_t)lda;
+  const size_t lds = sizeof(uint16_t) * 64 * (size_t)lda;   // the tile's activations: up to 128 KiB at F = 1016
   const dim3 grid((M + 63) / 64);
+  if (lds > 65536) {
+    hipFuncSetAttribute((const void*)mlp_start_fused_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipFuncSetAttribute((const void*)mlp_start_fused_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipFuncSetAttribute((const void*)mlp_start_fused_kernel<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipFuncSetAttribute((const void*)mlp_start_fused_kernel<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  }
   switch (N / 64) {
     case 1: hipLaunchKernelGGL((mlp_start_fused_kernel<1>), grid, dim3(256), lds, st, X, M, F, W1t, b1, W2t, b2, W3t, b3, N, inv_temp, thr); break;
     case 2: hipLaunchKernelGGL((mlp_start_fused_kernel<2>), grid, dim3(256), lds, st, X, M, F, W1t, b1, W2t, b2, W3t, b3, N, inv_temp, thr); break;

@@ -159,7 +159,7 @@ class WarmStartMLP:
         The kernels run on `stream` (default: torch's current stream), ordered after the work already queued there."""
         M, N = X.shape[0], self.N
         thr = out if out is not None else torch.empty(M, N, dtype=torch.int32, device=X.device)
-        if fused and self.H == 256 and N % 64 == 0 and N <= 256 and self.F <= 504:
+        if fused and self.H == 256 and N % 64 == 0 and N <= 256 and self.F <= 1016:
             W = self._weights_nmajor()
             P_ = self.params
             p = _lib.ptr

@@ -349,7 +349,7 @@ int arp_mlp_adam(float *W, const float *G, float *m, float *v, long n, float lr,
 /* fp32 -> bf16, round to nearest even. */
 int arp_cast_bf16(const float *x, long n, uint16_t *y, void *stream);
 /* Fused inference of the warm-start MLP (F -> H = 256 -> 256 -> N, GELU): features X (M x F bf16, F a multiple of 32
- * up to 504) and n-major bf16 weights W1t (H x F), W2t (H x H), W3t (N x H) with fp32 biases -> start thresholds
+ * up to 1016) and n-major bf16 weights W1t (H x F), W2t (H x H), W3t (N x H) with fp32 biases -> start thresholds
  * (M x N uint32, N in {64, 128, 192, 256}), equal bit for bit to three arp_mlp_gemm calls and arp_start_thresholds. */
 int arp_mlp_start_fused(const uint16_t *X, int M, int F, const uint16_t *W1t, const float *b1, const uint16_t *W2t,
                         const float *b2, const uint16_t *W3t, const float *b3, int H, int N, float inv_temp,

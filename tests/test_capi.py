@@ -138,6 +138,16 @@ def test_list_anneal_gram_forms_without_gpu():
     assert L.arp_abi_version() == 5
 
 
+def test_fused_mlp_feature_width_without_gpu():
+    """arp_mlp_start_fused takes feature rows up to 1016 wide (the tile's activations fit the 160 KiB of LDS)."""
+    from arpeggio_amd import _lib
+    L = _lib.load()
+    args = lambda F, N: (None, 4, F, None, None, None, None, None, None, 256, N, 1.0, None, None)
+    assert L.arp_mlp_start_fused(*args(1048, 256)) == -1 and b'32..1016' in L.arp_last_error()
+    assert L.arp_mlp_start_fused(*args(544, 320)) == -1 and b'N = 320' in L.arp_last_error()
+    assert L.arp_mlp_start_fused(*args(544, 256)) == -1 and b'null' in L.arp_last_error()
+
+
 def test_soft_argument_errors_without_gpu():
     from arpeggio_amd import _lib
     L = _lib.load()

@@ -183,11 +183,13 @@ def test_trained_start_gains_over_random_start():
 
 
 @pytest.mark.parametrize('M', [1, 63, 64, 1000])
-def test_fused_inference_equals_gemm_path(M):
-    """The fused features -> thresholds kernel gives the same bits as the three GEMMs + threshold kernel."""
-    from arpeggio_amd.learned import DEFAULT_WEIGHTS, WarmStartMLP
-    say('fused', M)
-    model = WarmStartMLP.load(DEFAULT_WEIGHTS)
+@pytest.mark.parametrize('net', ['32u', '64u'])
+def test_fused_inference_equals_gemm_path(M, net):
+    """The fused features -> thresholds kernel gives the same bits as the three GEMMs + threshold kernel: the 32-user
+    network (F = 288, N = 128) and the 64-user one (F = 544, N = 256: the tile's activations past 64 KiB of LDS)."""
+    from arpeggio_amd.learned import DEFAULT_WEIGHTS, WEIGHTS_64, WarmStartMLP
+    say('fused', M, net)
+    model = WarmStartMLP.load(DEFAULT_WEIGHTS if net == '32u' else WEIGHTS_64)
     rs = np.random.default_rng(M)
     X = torch.as_tensor(bf(rs.normal(size=(M, model.F)) * 0.7)).to(torch.bfloat16).cuda()
     X[:, 4 * model.D + 1:] = 0
@@ -198,7 +200,7 @@ def test_fused_inference_equals_gemm_path(M):
 
 
 def test_trained_start_gains_at_64_users():
-    """The 64-user network (Kronecker rho = 0.5, D = 128, unfused GEMM inference): on held-out frames at 28 dB the
+    """The 64-user network (Kronecker rho = 0.5, D = 128): on held-out frames at 28 dB the
     learned start at 64 passes beats the random start at 128 passes (fewer than half the passes), and at 32 passes its
     BER is less than half the random start's."""
     from arpeggio_amd import frames
