@@ -576,8 +576,9 @@ __device__ __forceinline__ void stage_gram(int32_t* __restrict__ sG, const int32
   }
 }
 
+// D = 64: eight waves per SIMD (64 registers; one value is spilled once per anneal), against seven at 66 registers.
 template <int P, int CPL, int MODE, bool FULL, int DS, int RNG>
-__global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt,
+__global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(DS == 64 ? 8 : 1))) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt,
                                                                                const int32_t* __restrict__ h,
                                                                                const int32_t* __restrict__ T, int L,
                                                                                int q, int R, int D,

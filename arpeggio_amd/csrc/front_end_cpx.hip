@@ -54,8 +54,9 @@ __device__ __forceinline__ uint32_t rne_bits(float x) { return __builtin_bit_cas
 
 }  // namespace
 
+// Four waves per SIMD (128 registers, no accumulator registers; at 16 users the kernel needs fewer anyway).
 template <int DP>
-__global__ __launch_bounds__(64) void front_end_cpx_kernel(const float* __restrict__ H, const float* __restrict__ y,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void front_end_cpx_kernel(const float* __restrict__ H, const float* __restrict__ y,
                                                            const float* __restrict__ N0v, float N0s, int M, int K,
                                                            float Q, FrontOut o) {
   constexpr int n = DP / 2, nC = n / 16, nU = nC * (nC + 1) / 2, Mr = DP, Nr = n, S = Mr / 4, SH = Nr / 4;
@@ -391,8 +392,10 @@ __global__ __launch_bounds__(64) void front_end_cpx_kernel(const float* __restri
 // k-steps taken in pairs (s, s + 16): rows r = 4 s + grp (top) and r + 64 (bottom) of C1, whose D1 rows are each other
 // (D1[r] = C1[r + 64], D1[r + 64] = -C1[r]). GIVEN: the MDI sub-problem build around given decisions (mimo.mdi's dDI
 // step, arpeggio/mimo/mimo.py:570-590): x_g read from xg_in, no inversion, workgroup i builds frame list[i].
+// Two waves per SIMD (256 registers, no accumulator registers): the sweep's serial pivot steps of one frame overlap
+// another frame's; the few values that do not fit are spilled once per frame, in the Gram phase.
 template <bool GIVEN>
-__global__ __launch_bounds__(64) void front_end_cpx128_kernel(const float* __restrict__ H, const float* __restrict__ y,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void front_end_cpx128_kernel(const float* __restrict__ H, const float* __restrict__ y,
                                                               const float* __restrict__ N0v, float N0s, int M, int K,
                                                               float Q, FrontOut o, const int8_t* __restrict__ xg_in,
                                                               const int32_t* __restrict__ list) {
