@@ -343,12 +343,17 @@ __device__ __forceinline__ void arpeggio_anneal(const int32_t* __restrict__ G4, 
       }
     }
   }
+  // the anneal's spin row from a wave-uniform base and a lane offset computed here (the volatile mbcnt pair keeps the
+  // compiler from hoisting a per-lane 64-bit address out of the pass loop, where it would hold two registers)
+  int8_t* const srow = out_s + t * (size_t)(P * D);
+  int lane_now;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane_now));
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    const int j = c * ARP_WAVE + lane;
+    const int j = c * ARP_WAVE + lane_now;
     if (valid[c]) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) out_s[t * (size_t)(P * D) + p * D + j] = Bs[p][c] ? -1 : 1;
+      for (int p = 0; p < P; ++p) srow[p * D + j] = Bs[p][c] ? -1 : 1;
     }
   }
   // flips = the wave's sum of the lane counts; tests = L + flips - flips on the last lane (D - 1)
@@ -576,7 +581,7 @@ __device__ __forceinline__ void stage_gram(int32_t* __restrict__ sG, const int32
   }
 }
 
-// D = 64: eight waves per SIMD (64 registers; one value is spilled once per anneal), against seven at 66 registers.
+// D = 64: eight waves per SIMD (at most 64 registers), against seven at 66.
 template <int P, int CPL, int MODE, bool FULL, int DS, int RNG>
 __global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(DS == 64 ? 8 : 1))) void arpeggio_fx_kernel(const int32_t* __restrict__ Gt,
                                                                                const int32_t* __restrict__ h,
@@ -598,7 +603,8 @@ __global__ __launch_bounds__(ARP_ARPEGGIO_MAX_THREADS) __attribute__((amdgpu_wav
   }
   stage_gram<CPL>(sG, Gt, ex.Gh, b, D);
   __syncthreads();
-  const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
+  // the wave index as a wave-uniform value: anneal index and output row in scalar registers
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), W = blockDim.x >> 6;
   const uint32_t in = inst[b];
   const uint32_t* thr = ex.thr ? ex.thr + (size_t)b * P * D : nullptr;
   for (int r = wv; r < R; r += W)
