@@ -82,25 +82,31 @@ def test_chord_f32_vs_oracle(orc, nt, P, kappa):
     assert np.abs(e.cpu().numpy() - E0).max() <= 1e-5 * np.abs(E0).max()
 
 
-def test_float_path_ber_in_bootstrap_ci(orc):
+@pytest.mark.parametrize('nt,cycles,snrs', [
+    (32, 64, (16.0, 20.0, 24.0)),
+    # 16 users x 16 rx, 16-QAM, 8 copies, 256 cycles, 4,096 frames per point, 10-20 dB in 2 dB steps
+    (16, 256, (10.0, 12.0, 14.0, 16.0, 18.0, 20.0)),
+])
+def test_float_path_ber_in_bootstrap_ci(orc, nt, cycles, snrs):
     """Per-use bit errors of the FP32 and the fixed-point Arpeggio chains on identical frames: the paired bootstrap
     CI of their BER ratio contains 1 at every SNR point."""
     from arpeggio_amd import arpeggiofx, frames
     from arpeggio_amd.detector import DetectPipeline, decode_select
-    B, nt, R = 4096, 32, 8
-    for snr in (16.0, 20.0, 24.0):
-        say('ber ci', snr)
+    B, R = 4096, 8
+    for snr in snrs:
+        say('ber ci', nt, snr)
         b = frames.draw(7, snr, 0, B, nt, 16, dev='cuda')
         lt = b['l'].to(torch.int8)
-        pipe = DetectPipeline(B, 2 * nt, 2 * nt, 16, R, policy='arpeggio', cycles=64, T0=0.1, T1=0.1, q=0.0, key=5)
+        pipe = DetectPipeline(B, 2 * nt, 2 * nt, 16, R, policy='arpeggio', cycles=cycles, T0=0.1, T1=0.1, q=0.0,
+                              key=5)
         _, be_fx = pipe.run(b['H'], b['y'], b['N0'], lt)
         from arpeggio_amd.detector import front_end
         o = front_end(b['H'], b['y'], b['N0'], 16, 1, want=('xg', 'Gf', 'hf'))
-        T = np.full(64, np.float32(0.1), dtype=np.float32)
+        T = np.full(cycles, np.float32(0.1), dtype=np.float32)
         s, _, _, _ = arpeggiofx.solve_float_device(o['Gf'], o['hf'], 2, R, T, 0.0, np.arange(B), 5, 'rand')
         _, be_f, _ = decode_select(b['H'], b['y'], o['xg'], s, 16, 2, True, ltrue=lt)
         torch.cuda.synchronize()
         a, c = be_f.cpu().numpy(), be_fx.cpu().numpy()
         r, lo, hi = orc.boot_ratio(a, c, B=2000)
-        say('ber', snr, a.sum() / (B * 128), c.sum() / (B * 128), r, lo, hi)
+        say('ber', nt, snr, a.sum() / (B * 4 * nt), c.sum() / (B * 4 * nt), r, lo, hi)
         assert lo <= 1.0 <= hi
