@@ -84,6 +84,8 @@ def parse():
     ap.add_argument('--batched-frames', type=int, default=1 << 17)
     ap.add_argument('--no-64u', action='store_true', help='skip the 64-user correlated-channel measurement')
     ap.add_argument('--frames-64u', type=int, default=1 << 16)
+    ap.add_argument('--sweep-frames', type=int, default=16384, help='frames per SNR point of the BER sweeps')
+    ap.add_argument('--no-sweep', action='store_true', help='skip the 32-user BER sweep over 12-22 dB')
     ap.add_argument('--copy-probe', action='store_true', help='run only a plain device copy of known size (PMC scale)')
     ap.add_argument('--no-mdi', action='store_true', help='skip the MDI wrapper against the reference fixtures')
     ap.add_argument('--mdi-batches', type=int, default=60, help='32-user fixture batches (1,000 frames each)')
@@ -551,6 +553,50 @@ def batched(args):
                 ber=det.bit_errors() / nbits(fr), R=8, passes=64, snr_db=[8.0, 24.0], graph=True)
 
 
+SWEEP_32 = (12.0, 14.0, 16.0, 18.0, 20.0, 22.0)     # the 32-user BER curve's SNR points (dB)
+SWEEP_64 = (14.0, 18.0, 22.0, 26.0)                 # the 64-user correlated channel's (dB)
+CHORD_32 = dict(T0=0.3, T1=0.2, q=0.4, kappa=0.3)   # the reference's x32 Chord schedule (fxdetect.py SCHED['x32'])
+
+
+def level_bit_errors(x, l):
+    """Per-frame Gray bit errors of 16-QAM real levels x against l (2 bits per level: sign and |x| == 1)."""
+    import torch
+    gb = lambda v: torch.stack([v > 0, v.abs() == 1], -1)
+    return (gb(x.float()) != gb(l.float())).flatten(1).sum(1).cpu().numpy().astype(np.int64)
+
+
+def sweep32(args, model, c, r2):
+    """BER against SNR at 32 users x 32 rx, 16-QAM, i.i.d. Rayleigh (the reference's draw recipe, draw index 2, so
+    none of these frames is a calibration or measurement frame): the LMMSE decision, plain first accept (16 x 64),
+    the learned adaptive detector at the headline's picks with its paired BER ratio to plain, and Chord at the
+    reference's x32 schedule (256 steps, 16 anneals)."""
+    from arpeggio_amd import frames
+    from arpeggio_amd.detector import DetectPipeline
+    from arpeggio_amd.stats import boot_ratio
+    Bs = args.sweep_frames
+    nb = Bs * 4 * NT
+    plain = make_pipe(Bs, args, args.passes, 'rand')
+    ap = make_adaptive(Bs, args, 'learned', model, c, r2)
+    ch = DetectPipeline(Bs, 2 * NT, 2 * NT, M, args.copies, policy='chord', cycles=256, key=key_of(args.seed), inst0=0,
+                        **CHORD_32)
+    rows = []
+    for snr in SWEEP_32:
+        b = frames.draw(args.seed, snr, 2, Bs, NT, M, dev='cuda')
+        ep = frame_errors(plain, b)
+        el = level_bit_errors(plain.xg, b['l'])
+        ea = frame_errors(ap, b)
+        ec = frame_errors(ch, b)
+        r, lo, hi = boot_ratio(ea, ep)
+        rows.append(dict(snr_db=snr, lmmse=float(el.sum()) / nb, plain=float(ep.sum()) / nb,
+                         adaptive=float(ea.sum()) / nb, adaptive_ratio_to_plain=r, ci=[lo, hi],
+                         open_fraction=int(ap.count.item()) / Bs, chord=float(ec.sum()) / nb))
+        log('sweep 32 users %.0f dB: LMMSE %.3e, plain %.3e, adaptive %.3e (ratio %.3f [%.3f, %.3f]), Chord %.3e' % (
+            snr, rows[-1]['lmmse'], rows[-1]['plain'], rows[-1]['adaptive'], r, lo, hi, rows[-1]['chord']))
+    return dict(frames_per_snr=Bs, users=NT, M=M, plain='%d x %d' % (args.copies, args.passes),
+                adaptive='stage 1 %s, stage 2 %d x %d, c %.1f' % (args.stage1, r2, args.passes, c),
+                chord='x32: %d x 256 steps, t 0.3 -> 0.2, q 0.4, kappa 0.3' % args.copies, points=rows)
+
+
 SNR_64 = 28.0                                       # where the annealers beat LMMSE on this channel
 SCHED_64 = dict(T0=0.08, T1=0.03, q=0.0)            # first accept, tuned at 26-28 dB on 4,096 frames
 CHORD_64 = dict(T0=0.2, T1=0.2, q=0.4, kappa=0.3)   # Chord re-tuned on the same frames (256 steps)
@@ -666,6 +712,25 @@ def users64(args):
     log('64 users, adaptive %d x %d then 32 x %d (c = %.1f): BER %.3e, ratio to plain 128 %.3f [%.3f, %.3f], %.2f ms'
         % (s1[0], s1[1], L2, c64, out['adaptive']['ber'], r, lo, hi, t['ms_per_step']))
     del pipe
+    # BER against SNR over 14-26 dB on separate draws: LMMSE, plain 32 x 128, the adaptive detector at the picks above
+    Bs, nb = args.sweep_frames, args.sweep_frames * 4 * nt
+    pl = DetectPipeline(Bs, 2 * nt, 2 * nt, 16, 32, key=key_of(args.seed), inst0=0, policy='arpeggio', cycles=128,
+                        start='rand', **SCHED_64)
+    ad = adaptive64(args, model, c64, s1, L2, Bs, nt)
+    sweep = []
+    for i, snr in enumerate(SWEEP_64):
+        bs = frames.draw_device(args.seed + 640 + i, snr, snr, Bs, nt, 16, rho_rx=0.5, rho_tx=0.5)
+        ep = frame_errors(pl, bs)
+        el = level_bit_errors(pl.xg, bs['l'])
+        ea = frame_errors(ad, bs)
+        r, lo, hi = boot_ratio(ea, ep)
+        sweep.append(dict(snr_db=snr, lmmse=float(el.sum()) / nb, plain_128=float(ep.sum()) / nb,
+                          adaptive=float(ea.sum()) / nb, adaptive_ratio_to_plain_128=r, ci=[lo, hi],
+                          open_fraction=int(ad.count.item()) / Bs))
+        log('sweep 64 users %.0f dB: LMMSE %.3e, plain 128 %.3e, adaptive %.3e (ratio %.3f [%.3f, %.3f])' % (
+            snr, sweep[-1]['lmmse'], sweep[-1]['plain_128'], sweep[-1]['adaptive'], r, lo, hi))
+    out['sweep'] = dict(frames_per_snr=Bs, points=sweep)
+    del pl, ad
     Bc = min(B, 4096)
     bc = {k: (v[:Bc] if torch.is_tensor(v) and v.dim() > 0 and v.shape[0] == B else v) for k, v in b.items()}
     pipe = DetectPipeline(Bc, 2 * nt, 2 * nt, 16, 32, key=key_of(args.seed), inst0=0, policy='chord', cycles=256,
@@ -814,6 +879,9 @@ def main():
     if not args.no_cpu:
         log('cpu baseline (3 segments)')
         out['cpu_baseline'] = cpu_baseline(b, plain, args)
+    if not args.no_sweep and model is not None:
+        log('32-user BER sweep (%d frames per SNR)' % args.sweep_frames)
+        out['sweep32'] = sweep32(args, model, c, r2)
     if not args.no_mdi:
         log('MDI wrapper against the reference fixtures')
         out['mdi'] = mdi(args)
