@@ -11,6 +11,10 @@
     python3 -m arpeggio_amd.profile_anneal chord [--frames 4096] [--launches 2]
         64 users x 64 rx, 16-QAM, Kronecker rho = 0.5, 28 dB: the bench's Chord detection (32 anneals of 256 steps,
         t 0.2 -> 0.2, q 0.4, kappa 0.3) `launches` times, for the snapshot kernel's counters.
+    python3 -m arpeggio_amd.profile_anneal users64 [--frames 65536] [--launches 2] [--stage1 8,32]
+        64 users x 64 rx, 16-QAM, Kronecker rho = 0.5, 28 dB (GPU-drawn frames): the bench's 64-user two-stage
+        detector (learned start from the 64-user network, T 0.08 -> 0.03, stage 2 32 anneals x 64 passes, c = 6)
+        `launches` times, for the counters of its front end, MLP, anneals and decodes.
     python3 -m arpeggio_amd.profile_anneal copy
         three device copies of 256 MiB: the known-size reference for scaling FETCH_SIZE / WRITE_SIZE.
     python3 -m arpeggio_amd.profile_anneal traffic FETCH_DB WRITE_DB COPY_FETCH_DB COPY_WRITE_DB [--match arpeggio]
@@ -103,6 +107,25 @@ def chord(a):
     print('chord launches', a.launches, 'bit errors', int(pipe.biterr.sum()))
 
 
+def users64(a):
+    import torch
+    from . import _lib, frames
+    from .detector import AdaptivePipeline
+    from .learned import WEIGHTS_64, WarmStartMLP
+    _lib.check(_lib.load().arp_set_front_end_form(2), 'front end form')
+    b = frames.draw_device(65, 28.0, 28.0, a.frames, 64, 16, rho_rx=0.5, rho_tx=0.5)
+    R1, L1 = (int(x) for x in a.stage1.split(','))
+    pipe = AdaptivePipeline(a.frames, 128, 128, 16, R1=R1, L1=L1, R2=32, L2=64, c=6.0, start='learned', key=1,
+                            model=WarmStartMLP.load(WEIGHTS_64), gate=False, T0=0.08, T1=0.03, q=0.0)
+    lt = b['l'].to(torch.int8).contiguous()
+    for i in range(a.launches):
+        pipe.run(b['H'], b['y'], b['N0'], lt)
+        torch.cuda.synchronize()
+        print('# step %d done' % i, file=sys.stderr, flush=True)
+    print(json.dumps(dict(frames=a.frames, stage1=a.stage1, open=int(pipe.count.item()),
+                          bit_errors=int(pipe.biterr.sum()))))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest='cmd', required=True)
@@ -121,6 +144,10 @@ def main():
     c = sub.add_parser('chord')
     c.add_argument('--frames', type=int, default=4096)
     c.add_argument('--launches', type=int, default=2)
+    u = sub.add_parser('users64')
+    u.add_argument('--frames', type=int, default=65536)
+    u.add_argument('--launches', type=int, default=2)
+    u.add_argument('--stage1', default='8,32', help='stage 1: anneals,passes (the bench\'s 64-user pick)')
     sub.add_parser('copy')
     t = sub.add_parser('traffic')
     t.add_argument('dbs', nargs=4)
@@ -131,6 +158,8 @@ def main():
         run(a)
     elif a.cmd == 'chord':
         chord(a)
+    elif a.cmd == 'users64':
+        users64(a)
     elif a.cmd == 'copy':
         copy()
     else:
